@@ -1,0 +1,201 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X range-predicate scan engine (one GPU).
+
+Default workload (the headline metric's configuration): the plain GPU scan at scale, one
+uint32 column of 1e8 uniform rows, 1000 single-range predicates with selectivity uniform
+in [0, 1], one query per launch (kernel a, rts_scan_range: bitmap + popcount).  A step is
+one predicate's scan of the whole column; --steps K times K of them back to back on one
+stream, bracketed by HIP events on that stream.
+
+Prints ONE JSON line: rows scanned per second, the dominant kernel's HBM roofline
+fraction (algorithmic bytes per launch / measured time per launch), and the CPU baseline
+(the oracle's multithreaded port of the reference raw_scan, timed on this host on a
+bounded sample of the same predicates).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "rows scanned/s & % HBM peak (1e8 uint32); batched queries/s; p50/p99 latency"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+N_ROWS = 100_000_000
+N_PRED = 1000
+DATA_SEED = 2
+PRED_SEED = 22
+
+
+def config2_predicates(count=N_PRED, seed=PRED_SEED):
+    """Closed ranges with selectivity uniform in [0, 1]: width = round(sel * 2^32), lo
+    uniform on [0, 2^32 - width] (so hi never clips)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        width = max(1, int(round(float(rng.random()) * 2.0 ** 32)))
+        lo = int(rng.integers(0, 2 ** 32 - width + 1))
+        out.append((lo, lo + width - 1))
+    return out
+
+
+def load_traffic(name):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE, scaled by
+    the copy-kernel calibration), from the committed profile summary, or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.isfile(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            try:
+                return float(line.split(":", 1)[1].strip())
+            except ValueError:
+                return None
+    return None
+
+
+def reference_predicate(lo, hi):
+    """The reference query line for the closed range [lo, hi]: bt (open) inside the domain,
+    le / ge at its edges."""
+    import oracle
+
+    if lo == 0 and hi == 0xFFFFFFFF:
+        return oracle.GE, 0, 0
+    if lo == 0:
+        return oracle.LE, hi, 0
+    if hi == 0xFFFFFFFF:
+        return oracle.GE, lo, 0
+    return oracle.BT, lo - 1, hi + 1
+
+
+def cpu_baseline(x, preds, threads, budget_s=1.5, max_preds=24):
+    """The oracle's port of raw_scan (bindex.cpp:1851-1892), word-parallel over `threads`
+    host threads, on the first predicates of the workload until ~budget_s of wall time."""
+    import oracle
+
+    n = x.size
+    done, t_total, counts = 0, 0.0, []
+    for lo, hi in preds[:max_preds]:
+        bm = np.zeros((n + 31) // 32, np.uint32)
+        op, t1, t2 = reference_predicate(lo, hi)
+        t0 = time.perf_counter()
+        oracle.raw_scan(x, op, t1, t2, bitmap=bm, threads=threads)
+        t_total += time.perf_counter() - t0
+        counts.append(oracle.popcount_rows(bm, n))
+        done += 1
+        if t_total >= budget_s:
+            break
+    return {"rows_per_s": done * n / t_total, "preds": done, "wall_s": t_total,
+            "counts": counts}
+
+
+def bench_scan(args, R):
+    n = N_ROWS
+    t0 = time.time()
+    x = R.generate_column(R.DIST_UNIFORM, DATA_SEED, 0, n)
+    dx = R.DeviceBuffer.from_numpy(x)
+    preds = config2_predicates()
+    out = R.DeviceBuffer(4 * ((n + 31) // 32))
+    steps, warmup = args.steps, args.warmup
+    counts = R.DeviceBuffer(8 * max(steps, 1))
+    stream = R.Stream()
+    e0, e1 = R.Event(), R.Event()
+
+    class Slot:
+        def __init__(self, i):
+            self.ptr = counts.ptr + 8 * i
+
+    for i in range(warmup):
+        lo, hi = preds[i % len(preds)]
+        R.scan_range(dx, n, lo, hi, out, Slot(i % max(steps, 1)), stream)
+    stream.synchronize()
+    R.synchronize()
+    e0.record(stream)
+    for i in range(steps):
+        lo, hi = preds[i % len(preds)]
+        R.scan_range(dx, n, lo, hi, out, Slot(i), stream)
+    e1.record(stream)
+    e1.synchronize()
+    R.synchronize()
+    ms = e0.elapsed_ms(e1)
+    gpu_counts = counts.to_numpy(np.uint64, steps)
+
+    per_launch_s = ms / 1e3 / steps
+    alg_bytes = n * 4 + ((n + 31) // 32) * 4  # column read + bitmap write
+    achieved = alg_bytes / per_launch_s / 1e9
+    traffic = load_traffic("scan_range_1e8_traffic.txt")
+
+    cores = min(16, os.cpu_count() or 1)
+    cpu = cpu_baseline(x, preds[:steps] if steps else preds, cores)
+    ok = all(int(gpu_counts[i]) == c for i, c in enumerate(cpu["counts"]) if i < steps)
+
+    return {
+        "metric": METRIC,
+        "value": n * steps / (ms / 1e3),
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": warmup,
+        "ms_per_step": ms / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic (seeded uniform uint32 column, seed %d)" % DATA_SEED,
+        "config": {
+            "workload": "plain GPU scan: 1 uint32 column x 1e8 rows, 1000 single-range "
+                        "predicates with selectivity U[0,1], one query per launch "
+                        "(rts_scan_range, bitmap + popcount)",
+            "rows": n,
+            "predicates": N_PRED,
+            "kernel": "k_scan_tiles<1,4,0,true>",
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+        },
+        "cpu_baseline": {
+            "value": cpu["rows_per_s"],
+            "unit": "rows/s",
+            "cores": cores,
+            "kind": "port",
+            "sample": "oracle raw_scan port (bindex.cpp:1851-1892), %d threads, first %d of "
+                      "the 1000 predicates over the full 1e8-row column (%.2f s wall)"
+                      % (cores, cpu["preds"], cpu["wall_s"]),
+        },
+        "parity": {"checked_predicates": len(cpu["counts"]), "popcounts_match": bool(ok)},
+        "setup_s": round(time.time() - t0, 2),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="scan", choices=["scan"])
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
+              "(row sharding across GPUs is described in DESIGN.md, not run)" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import rtscan_amd as R
+
+    if R.lib().rts_device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    res = bench_scan(args, R)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * rtscan_hip.h -- C ABI of the MI355X range-predicate scan engine (librtscan_hip.so).
+ *
+ * Drop-in surface: the names, argument order and units of the reference's GPU call
+ * surface (bindex.h:145-183, cuda_and.cu, rt.h:142-145), with hipError_t in place of
+ * cudaError_t.  Additive surface: rts_* entry points (direct scan, fused conjunction,
+ * AND+popcount, device-memory and timing helpers, seeded column generators).
+ *
+ * Conventions shared by every entry point:
+ *   - Columns are uint32 (CODE) arrays in device memory, one value per row.
+ *   - Bitmaps are 1 bit per row, MSB-first inside 32-bit words: row i is bit
+ *     (1u << (31 - i % 32)) of word i / 32 (bindex.cpp:269).  rts_* entry points write
+ *     ceil(n / 32) words and leave the bits past row n at 0.
+ *   - Reference-named wrappers run on the legacy null stream and return after the work
+ *     has finished (the reference ends every wrapper with cudaDeviceSynchronize,
+ *     cuda_and.cu:276-287).  They use the calling thread's current device instead of
+ *     forcing device 0 (cuda_and.cu:109).  rts_* entry points are asynchronous on the
+ *     stream they are given.
+ *   - Errors are returned, never fatal: no entry point exits the process.
+ */
+#ifndef RTSCAN_HIP_H_
+#define RTSCAN_HIP_H_
+
+#include <hip/hip_runtime_api.h>
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- types (bindex.h:59-100) */
+typedef uint32_t CODE;     /* bindex.h:59 */
+typedef int POSTYPE;       /* bindex.h:86 */
+typedef unsigned int BITS; /* bindex.h:87 */
+
+/* bindex.h:88-95 */
+enum OPERATOR {
+  EQ = 0, /* x == a */
+  LT,     /* x <  a */
+  GT,     /* x >  a */
+  LE,     /* x <= a */
+  GE,     /* x >= a */
+  BT      /* a < x < b (open interval; a > b is swapped by the query driver) */
+};
+
+/* ------------------------------------------------------ cuda_and.cu surface (bitmap ops) */
+
+/* cuda_and.cu:255-296: a[i] &= b[i] over bits_num_needed(n) words; n counts ROWS. */
+hipError_t GPUbitAndWithCuda(BITS *dev_bitmap_a, BITS *dev_bitmap_b, unsigned int n);
+/* cuda_and.cu:298-338: a[i] |= b[i]; n counts ROWS. */
+hipError_t GPUbitOrWithCuda(BITS *dev_bitmap_a, BITS *dev_bitmap_b, unsigned int n);
+/* cuda_and.cu:340-380: a[i] = b[i]; n counts WORDS. */
+hipError_t GPUbitCopyWithCuda(BITS *dev_bitmap_a, BITS *dev_bitmap_b, unsigned int n);
+/* cuda_and.cu:382-422: a[i] = ~b[i]; n counts WORDS. */
+hipError_t GPUbitCopyNegationWithCuda(BITS *dev_bitmap_a, BITS *dev_bitmap_b, unsigned int n);
+/* cuda_and.cu:424-463: result[i] = b[i] & ~a[i]; bitnum counts WORDS. */
+hipError_t GPUbitCopySIMDWithCuda(BITS *result, BITS *dev_bitmap_a, BITS *dev_bitmap_b,
+                                  unsigned int bitnum);
+/* cuda_and.cu:468-508 (defined, not declared, in the reference): a[i] = value over
+ * bits_num_needed(n) words; n counts ROWS. */
+hipError_t GPUbitMemsetWithCuda(BITS *dev_bitmap_a, BITS value, unsigned int n);
+/* cuda_and.cu:168-253: host bitmaps; a &= b over bits_num_needed(size) words. */
+hipError_t CPUbitAndWithCuda(BITS *bitmap_a, BITS *bitmap_b, unsigned int size);
+/* cuda_and.cu:101-166: flips rows 0..size-1 of a host bitmap of `size` words.  The
+ * reference kernel (cuda_and.cu:3-10) races 32 threads on one word without atomics; this
+ * one flips every bit exactly once. */
+hipError_t refineWithCuda(CODE *bitmap, unsigned int size);
+
+/* ------------------------------------------------------------- bindex.h surface (scan) */
+
+/* raw_scan (bindex.h:146, bindex.cpp:1851-1892): the reference's ground-truth scan.  Every
+ * row of the HOST column `raw_data` (length bindex->length) that satisfies
+ * `OP` (t1, t2) XOR-toggles its bit in the HOST bitmap `bitmap`.  With compare_bitmap
+ * non-NULL the reference only inspects and leaves `bitmap` unchanged; so does this one.
+ * Evaluated on the GPU (upload, scan kernel in XOR mode, download).  The BinDex argument
+ * is only read for its `length`; see rts_bindex.h for the struct. */
+struct BinDex;
+void raw_scan(struct BinDex *bindex, BITS *bitmap, CODE target1, CODE target2,
+              enum OPERATOR OP, CODE *raw_data, BITS *compare_bitmap);
+
+/* ---------------------------------------------------------- additive: device helpers */
+
+hipError_t rts_malloc(void **ptr, size_t bytes);
+hipError_t rts_free(void *ptr);
+hipError_t rts_memcpy_htod(void *dst, const void *src, size_t bytes);
+hipError_t rts_memcpy_dtoh(void *dst, const void *src, size_t bytes);
+hipError_t rts_memcpy_dtod(void *dst, const void *src, size_t bytes);
+hipError_t rts_memset(void *dst, int byte_value, size_t bytes);
+hipError_t rts_device_synchronize(void);
+hipError_t rts_stream_create(hipStream_t *stream);
+hipError_t rts_stream_destroy(hipStream_t stream);
+hipError_t rts_stream_synchronize(hipStream_t stream);
+hipError_t rts_event_create(hipEvent_t *event);
+hipError_t rts_event_destroy(hipEvent_t event);
+hipError_t rts_event_record(hipEvent_t event, hipStream_t stream);
+hipError_t rts_event_synchronize(hipEvent_t event);
+hipError_t rts_event_elapsed_ms(float *ms, hipEvent_t start, hipEvent_t stop);
+/* Device count, name and compute-unit count of the current device. */
+int rts_device_count(void);
+hipError_t rts_device_info(char *name, size_t name_len, int *cu_count, size_t *total_mem);
+/* Message of the last error recorded by this library on the calling thread. */
+const char *rts_last_error(void);
+
+/* ------------------------------------------------------------ additive: scan kernels */
+
+/* Normalise a reference predicate (OPERATOR, t1, t2) to a closed interval [*lo, *hi];
+ * returns 0 when no uint32 satisfies it (empty), 1 otherwise.  BT is the open interval of
+ * the swapped bounds (bindex.cpp:1903-1905, 1857-1874). */
+int rts_predicate_interval(enum OPERATOR op, CODE t1, CODE t2, CODE *lo, CODE *hi);
+
+/* Kernel (a): direct streaming compare of one device column against the closed interval
+ * [lo, hi] (lo > hi means empty), writing the n-row bitmap `out` (ceil(n/32) words, tail
+ * bits 0).  `count` (device, may be NULL) receives the number of matching rows. */
+hipError_t rts_scan_range(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
+                          unsigned long long *count, hipStream_t stream);
+
+/* As rts_scan_range, but XOR-toggles the matching rows into `out` (raw_scan semantics). */
+hipError_t rts_scan_range_xor(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
+                              hipStream_t stream);
+
+/* Kernel (c), direct form: AND over `ncols` columns of closed-interval predicates and a
+ * popcount in one pass (no per-column bitmaps).  `columns`, `lo`, `hi` are host arrays of
+ * ncols entries (device column pointers, bounds).  ncols >= 1. */
+hipError_t rts_conjunction_scan(const CODE *const *columns, const CODE *lo, const CODE *hi,
+                                int ncols, uint64_t n, BITS *out, unsigned long long *count,
+                                hipStream_t stream);
+
+/* Kernel (c), bitmap form: out = AND of `nbitmaps` n-row device bitmaps, popcount of the
+ * first n rows into `count` (may be NULL), one pass.  Replaces the chain of
+ * GPUbitAndWithCuda calls in merge_with_GPU (bindex_cuda.cpp:1693-1702) plus the host
+ * hit count.  `bitmaps` is a host array; `out` may alias bitmaps[0]. */
+hipError_t rts_bitmap_and_popcount(const BITS *const *bitmaps, int nbitmaps, uint64_t n,
+                                   BITS *out, unsigned long long *count, hipStream_t stream);
+
+/* Popcount of the first n rows of a device bitmap into *count (device). */
+hipError_t rts_bitmap_popcount(const BITS *bitmap, uint64_t n, unsigned long long *count,
+                               hipStream_t stream);
+
+/* -------------------------------------------------- additive: seeded column generators */
+
+/* Distributions of the synthetic workloads. */
+enum rts_dist {
+  RTS_DIST_UNIFORM = 0, /* uniform on [0, 2^32) */
+  RTS_DIST_NORMAL = 1,  /* round(N(p0, p1)) clipped to [0, 2^32-1] */
+  RTS_DIST_ZIPF = 2     /* Zipf(s = p0) over ranks 1..p1, rank -> distinct random key */
+};
+
+/* Fill host array `out` (n values) with column `column_id` of a seeded table.  The value
+ * of row i depends only on (dist, seed, column_id, i, p0, p1), so any thread split gives
+ * the same column.  Returns 0, or -1 on bad parameters. */
+int rts_generate_column(int dist, uint64_t seed, int column_id, uint64_t n, double p0, double p1,
+                        CODE *out, int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTSCAN_HIP_H_ */

@@ -1,0 +1,87 @@
+/*
+ * oracle.h -- CPU restatement of the reference's scan / BinDex path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the shipped engine (rtscan_amd/, include/)
+ * includes, links or calls this code.  It is used by tests/ (as the checker),
+ * by __graft_entry__.smoke() (as the checker) and by bench.py's cpu_baseline leg
+ * (as the timed CPU port of the reference path).
+ *
+ * Every function cites the reference file:line it restates.  Reference paths are
+ * relative to the RTScan repository root.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   orc_raw_scan / _mt / _entry   closed-form numpy predicates on random and edge inputs,
+ *                                 hand-written bit-order fixtures, and the compiled
+ *                                 reference raw_scan (oracle/_ref) on the same inputs.
+ *   orc_compare_bitmap            hand-computed counts on crafted bitmaps.
+ *   orc_init_bindex               compiled reference init_bindex (oracle/_ref): area start
+ *                                 values, area counts and all 127 filter vectors bit-exact;
+ *                                 closed form fv[k] = (x < start[k+1]) in numpy.
+ *   orc_in_which_area/_block,     compiled reference on the same BinDex.
+ *   orc_on_which_pos
+ *   orc_bindex_scan               compiled reference bindex_scan_{lt,le,gt,ge,bt,eq}
+ *                                 bit-exact on every tested query (including the inputs
+ *                                 where the reference disagrees with its own raw_scan),
+ *                                 and raw_scan on the inputs where the reference is right.
+ */
+#ifndef RTSCAN_ORACLE_H_
+#define RTSCAN_ORACLE_H_
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* bindex.h:88-95 */
+enum orc_op { ORC_EQ = 0, ORC_LT = 1, ORC_GT = 2, ORC_LE = 3, ORC_GE = 4, ORC_BT = 5 };
+
+/* bindex.h:104 (blockInitSize) */
+#define ORC_BLOCK_INIT_SIZE 3276
+
+int orc_bits_num_needed(int n);
+
+void orc_raw_scan(const uint32_t *raw, int n, uint32_t *bitmap, uint32_t t1, uint32_t t2, int op);
+void orc_raw_scan_mt(const uint32_t *raw, int n, uint32_t *bitmap, uint32_t t1, uint32_t t2, int op,
+                     int threads);
+void orc_raw_scan_entry(const uint32_t *raw, int n, int op, const uint32_t *tl, const uint32_t *tr,
+                        int ntargets, uint32_t *bitmap, uint32_t *merge_bitmap, int threads);
+void orc_compare_bitmap(const uint32_t *truth, const uint32_t *got, int n, int *total_hit,
+                        int *true_hit, int *first_mismatch);
+uint64_t orc_popcount_rows(const uint32_t *bitmap, int n);
+
+/* Flat restatement of the reference BinDex (bindex.cpp:275-298): areas are contiguous
+ * ranges of the value-sorted arrays, blocks are ORC_BLOCK_INIT_SIZE-row slices of an area. */
+typedef struct {
+  int n;
+  int K;
+  int words;              /* bits_num_needed(n) */
+  uint32_t *sorted_val;   /* n, ascending (stable argsort) */
+  int32_t *sorted_pos;    /* n */
+  uint32_t *start_values; /* K: area_start_value(areas[k]) */
+  int32_t *area_start;    /* K: areaStartIdx */
+  int32_t *area_len;      /* K */
+  int32_t *area_counts;   /* K: prefix sums (bindex.cpp:649-652) */
+  uint32_t *fvs;          /* (K-1) * words: fv[k] = x < start_values[k+1] */
+  uint32_t data_min;
+  uint32_t data_max;
+} orc_bindex;
+
+int orc_init_bindex(orc_bindex *bx, const uint32_t *data, int n, int K, int threads);
+void orc_free_bindex(orc_bindex *bx);
+
+int orc_area_block_num(const orc_bindex *bx, int area);
+int orc_in_which_area(const orc_bindex *bx, uint32_t c);
+int orc_in_which_block(const orc_bindex *bx, int area, uint32_t c);
+int orc_on_which_pos(const orc_bindex *bx, int area, int block, uint32_t c);
+
+/* One reference BinDex scan (bindex_scan_lt/le/gt/ge/bt/eq), written into `result`.
+ * Returns 0, or -1 where the reference asserts (bt with c2 <= c1, bt whose upper
+ * bound is below the data, eq outside the data, eq whose xor path asserts). */
+int orc_bindex_scan(const orc_bindex *bx, uint32_t *result, int op, uint32_t t1, uint32_t t2,
+                    int threads);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,267 @@
+"""Python front end of the MI355X range-predicate scan engine.
+
+A thin ctypes layer over ``librtscan_hip.so`` (the C ABI declared in include/*.h): the
+work runs in the library's HIP kernels.  There is no fallback: when the library is not
+built, or no GPU is present, the entry points raise.
+
+Bitmaps are 1 bit per row, MSB-first in uint32 words (row i is bit 1 << (31 - i % 32)
+of word i // 32), as in the reference (bindex.cpp:269).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librtscan_hip.so")
+
+EQ, LT, GT, LE, GE, BT = 0, 1, 2, 3, 4, 5  # enum OPERATOR, bindex.h:88-95
+OPS = {"eq": EQ, "lt": LT, "gt": GT, "le": LE, "ge": GE, "bt": BT}
+DIST_UNIFORM, DIST_NORMAL, DIST_ZIPF = 0, 1, 2
+
+_vp = ctypes.c_void_p
+_u32 = ctypes.c_uint32
+_u64 = ctypes.c_uint64
+_int = ctypes.c_int
+_err = ctypes.c_int  # hipError_t
+
+# name -> (restype, argtypes) for every function the headers in include/ declare
+SIGNATURES = {
+    # cuda_and.cu surface
+    "GPUbitAndWithCuda": (_err, [_vp, _vp, ctypes.c_uint]),
+    "GPUbitOrWithCuda": (_err, [_vp, _vp, ctypes.c_uint]),
+    "GPUbitCopyWithCuda": (_err, [_vp, _vp, ctypes.c_uint]),
+    "GPUbitCopyNegationWithCuda": (_err, [_vp, _vp, ctypes.c_uint]),
+    "GPUbitCopySIMDWithCuda": (_err, [_vp, _vp, _vp, ctypes.c_uint]),
+    "GPUbitMemsetWithCuda": (_err, [_vp, _u32, ctypes.c_uint]),
+    "CPUbitAndWithCuda": (_err, [_vp, _vp, ctypes.c_uint]),
+    "refineWithCuda": (_err, [_vp, ctypes.c_uint]),
+    # bindex.h surface
+    "raw_scan": (None, [_vp, _vp, _u32, _u32, _int, _vp, _vp]),
+    # helpers
+    "rts_malloc": (_err, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "rts_free": (_err, [_vp]),
+    "rts_memcpy_htod": (_err, [_vp, _vp, ctypes.c_size_t]),
+    "rts_memcpy_dtoh": (_err, [_vp, _vp, ctypes.c_size_t]),
+    "rts_memcpy_dtod": (_err, [_vp, _vp, ctypes.c_size_t]),
+    "rts_memset": (_err, [_vp, _int, ctypes.c_size_t]),
+    "rts_device_synchronize": (_err, []),
+    "rts_stream_create": (_err, [ctypes.POINTER(_vp)]),
+    "rts_stream_destroy": (_err, [_vp]),
+    "rts_stream_synchronize": (_err, [_vp]),
+    "rts_event_create": (_err, [ctypes.POINTER(_vp)]),
+    "rts_event_destroy": (_err, [_vp]),
+    "rts_event_record": (_err, [_vp, _vp]),
+    "rts_event_synchronize": (_err, [_vp]),
+    "rts_event_elapsed_ms": (_err, [ctypes.POINTER(ctypes.c_float), _vp, _vp]),
+    "rts_device_count": (_int, []),
+    "rts_device_info": (_err, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_int),
+                               ctypes.POINTER(ctypes.c_size_t)]),
+    "rts_last_error": (ctypes.c_char_p, []),
+    # scans
+    "rts_predicate_interval": (_int, [_int, _u32, _u32, ctypes.POINTER(_u32),
+                                      ctypes.POINTER(_u32)]),
+    "rts_scan_range": (_err, [_vp, _u64, _u32, _u32, _vp, _vp, _vp]),
+    "rts_scan_range_xor": (_err, [_vp, _u64, _u32, _u32, _vp, _vp]),
+    "rts_conjunction_scan": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
+    "rts_bitmap_and_popcount": (_err, [_vp, _int, _u64, _vp, _vp, _vp]),
+    "rts_bitmap_popcount": (_err, [_vp, _u64, _vp, _vp]),
+    # generators
+    "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
+                                   _vp, _int]),
+}
+
+_lib = None
+
+
+class HipError(RuntimeError):
+    pass
+
+
+def lib():
+    """Load librtscan_hip.so; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.isfile(LIB_PATH):
+            raise ImportError("librtscan_hip.so is not built: run __graft_entry__.build() "
+                              "or make -C rtscan_amd/csrc")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(err, what=""):
+    if err:
+        msg = lib().rts_last_error().decode(errors="replace")
+        raise HipError("%s failed: hipError %d %s" % (what, err, msg))
+
+
+def gpu_available():
+    try:
+        return lib().rts_device_count() > 0
+    except (ImportError, OSError):
+        return False
+
+
+def words_for(n):
+    return (n + 31) // 32
+
+
+def predicate_interval(op, t1, t2=0):
+    """(lo, hi) closed interval of a reference predicate, or None when empty."""
+    lo, hi = _u32(), _u32()
+    ok = lib().rts_predicate_interval(op, t1 & 0xFFFFFFFF, t2 & 0xFFFFFFFF, ctypes.byref(lo),
+                                      ctypes.byref(hi))
+    return (lo.value, hi.value) if ok else None
+
+
+class DeviceBuffer:
+    """Device allocation owned by Python (rts_malloc / rts_free)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = _vp()
+        check(lib().rts_malloc(ctypes.byref(p), max(self.nbytes, 4)), "rts_malloc")
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, arr):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        if arr.nbytes:
+            check(lib().rts_memcpy_htod(b.ptr, arr.ctypes.data, arr.nbytes), "htod")
+        return b
+
+    def to_numpy(self, dtype=np.uint32, count=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype)
+        if out.nbytes:
+            check(lib().rts_memcpy_dtoh(out.ctypes.data, self.ptr, out.nbytes), "dtoh")
+        return out
+
+    def fill(self, byte_value=0):
+        check(lib().rts_memset(self.ptr, byte_value, self.nbytes), "memset")
+        return self
+
+    def free(self):
+        if self.ptr:
+            lib().rts_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Stream:
+    def __init__(self):
+        s = _vp()
+        check(lib().rts_stream_create(ctypes.byref(s)), "stream_create")
+        self.handle = s.value
+
+    def synchronize(self):
+        check(lib().rts_stream_synchronize(self.handle), "stream_synchronize")
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().rts_stream_destroy(self.handle)
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        e = _vp()
+        check(lib().rts_event_create(ctypes.byref(e)), "event_create")
+        self.handle = e.value
+
+    def record(self, stream=None):
+        check(lib().rts_event_record(self.handle, stream.handle if stream else None), "record")
+
+    def synchronize(self):
+        check(lib().rts_event_synchronize(self.handle), "event_synchronize")
+
+    def elapsed_ms(self, later):
+        ms = ctypes.c_float()
+        check(lib().rts_event_elapsed_ms(ctypes.byref(ms), self.handle, later.handle), "elapsed")
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.handle:
+                lib().rts_event_destroy(self.handle)
+        except Exception:
+            pass
+
+
+def device_info():
+    name = ctypes.create_string_buffer(128)
+    cus, mem = _int(), ctypes.c_size_t()
+    check(lib().rts_device_info(name, 128, ctypes.byref(cus), ctypes.byref(mem)), "device_info")
+    return {"arch": name.value.decode(), "cus": cus.value, "total_mem": mem.value}
+
+
+def _sh(stream):
+    return stream.handle if stream is not None else None
+
+
+def scan_range(col, n, lo, hi, out, count=None, stream=None):
+    """Kernel (a): bitmap of rows of device column `col` with lo <= x <= hi."""
+    check(lib().rts_scan_range(col.ptr, n, lo, hi, out.ptr, count.ptr if count else None,
+                               _sh(stream)), "rts_scan_range")
+
+
+def scan_range_xor(col, n, lo, hi, out, stream=None):
+    check(lib().rts_scan_range_xor(col.ptr, n, lo, hi, out.ptr, _sh(stream)),
+          "rts_scan_range_xor")
+
+
+def _ptr_array(ptrs):
+    arr = (_vp * len(ptrs))(*ptrs)
+    return arr
+
+
+def conjunction_scan(cols, los, his, n, out, count=None, stream=None):
+    """Kernel (c), direct form: AND over columns of lo_c <= x_c <= hi_c, plus popcount."""
+    k = len(cols)
+    pa = _ptr_array([c.ptr for c in cols])
+    lo = np.ascontiguousarray(los, np.uint32)
+    hi = np.ascontiguousarray(his, np.uint32)
+    check(lib().rts_conjunction_scan(pa, lo.ctypes.data, hi.ctypes.data, k, n, out.ptr,
+                                     count.ptr if count else None, _sh(stream)),
+          "rts_conjunction_scan")
+
+
+def bitmap_and_popcount(bitmaps, n, out, count=None, stream=None):
+    pa = _ptr_array([b.ptr for b in bitmaps])
+    check(lib().rts_bitmap_and_popcount(pa, len(bitmaps), n, out.ptr,
+                                        count.ptr if count else None, _sh(stream)),
+          "rts_bitmap_and_popcount")
+
+
+def bitmap_popcount(bitmap, n, count, stream=None):
+    check(lib().rts_bitmap_popcount(bitmap.ptr, n, count.ptr, _sh(stream)), "rts_bitmap_popcount")
+
+
+def synchronize():
+    check(lib().rts_device_synchronize(), "device_synchronize")
+
+
+def generate_column(dist, seed, column_id, n, p0=0.0, p1=0.0, threads=None):
+    """Seeded synthetic column (host numpy array) from the library's generator."""
+    out = np.empty(n, np.uint32)
+    if threads is None:
+        threads = min(16, os.cpu_count() or 1)
+    rc = lib().rts_generate_column(dist, seed, column_id, n, p0, p1, out.ctypes.data, threads)
+    if rc:
+        raise ValueError("rts_generate_column: bad parameters")
+    return out

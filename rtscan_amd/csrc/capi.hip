@@ -1,0 +1,303 @@
+// capi.hip -- extern "C" entry points: the reference-named cuda_and.cu / bindex.h
+// wrappers and the additive rts_* scan, memory and timing entry points.
+#include <stdio.h>
+#include <string.h>
+
+#include <map>
+#include <mutex>
+#include <thread>
+#include <tuple>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "rts_bindex.h"
+
+namespace rts {
+
+hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE *hi, int ncols,
+                            uint64_t n, BITS *out, unsigned long long *count, int mode,
+                            hipStream_t s);
+hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS *out,
+                               unsigned long long *count, hipStream_t s);
+hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
+                 hipStream_t s);
+hipError_t flip_rows(BITS *bm, uint64_t rows, hipStream_t s);
+
+enum { kCopy = 0, kNot = 1, kAndNot = 2, kAnd = 3, kOr = 4, kFill = 5, kXorWith = 6 };
+
+static thread_local std::string g_last_error;
+
+hipError_t set_error(hipError_t e, const char *what) {
+  if (e != hipSuccess) {
+    char buf[256];
+    snprintf(buf, sizeof(buf), "%s: %s (%d)", what, hipGetErrorString(e), (int)e);
+    g_last_error = buf;
+  }
+  return e;
+}
+
+int device_cu_count() {
+  static std::mutex mu;
+  static std::vector<int> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 256;
+  std::lock_guard<std::mutex> lock(mu);
+  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
+  if (cache[dev] == 0) {
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    cache[dev] = cus;
+  }
+  return cache[dev];
+}
+
+CountScratch *stream_count_scratch(hipStream_t stream) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, uintptr_t, std::thread::id>, CountScratch *> table;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
+  // hipStreamPerThread names a different stream on every host thread
+  const std::thread::id tid =
+      stream == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id();
+  const auto key = std::make_tuple(dev, (uintptr_t)stream, tid);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = table.find(key);
+  if (it != table.end()) return it->second;
+  CountScratch *p = nullptr;
+  if (hipMalloc((void **)&p, 256) != hipSuccess) return nullptr;
+  if (hipMemset(p, 0, 256) != hipSuccess || hipStreamSynchronize(0) != hipSuccess) return nullptr;
+  table[key] = p;
+  return p;
+}
+
+// Reference wrappers: legacy null stream, synchronous (cuda_and.cu ends every wrapper with
+// cudaDeviceSynchronize); launch errors are returned, never fatal.
+static hipError_t finish_sync(hipError_t e, const char *what) {
+  if (e != hipSuccess) return set_error(e, what);
+  return set_error(hipStreamSynchronize(0), what);
+}
+
+}  // namespace rts
+
+using namespace rts;
+
+extern "C" {
+
+// ---------------------------------------------------------------- cuda_and.cu surface
+
+hipError_t GPUbitAndWithCuda(BITS *a, BITS *b, unsigned int n) {
+  return finish_sync(bitop(kAnd, a, b, nullptr, ref_words((int)n), 0, 0), "GPUbitAndWithCuda");
+}
+
+hipError_t GPUbitOrWithCuda(BITS *a, BITS *b, unsigned int n) {
+  return finish_sync(bitop(kOr, a, b, nullptr, ref_words((int)n), 0, 0), "GPUbitOrWithCuda");
+}
+
+hipError_t GPUbitCopyWithCuda(BITS *a, BITS *b, unsigned int n) {
+  return finish_sync(bitop(kCopy, a, b, nullptr, n, 0, 0), "GPUbitCopyWithCuda");
+}
+
+hipError_t GPUbitCopyNegationWithCuda(BITS *a, BITS *b, unsigned int n) {
+  return finish_sync(bitop(kNot, a, b, nullptr, n, 0, 0), "GPUbitCopyNegationWithCuda");
+}
+
+// result = b & ~a (bitmap_copySIMDKernel, cuda_and.cu:54-61)
+hipError_t GPUbitCopySIMDWithCuda(BITS *result, BITS *a, BITS *b, unsigned int bitnum) {
+  return finish_sync(bitop(kAndNot, result, a, b, bitnum, 0, 0), "GPUbitCopySIMDWithCuda");
+}
+
+hipError_t GPUbitMemsetWithCuda(BITS *a, BITS value, unsigned int n) {
+  return finish_sync(bitop(kFill, a, nullptr, nullptr, ref_words((int)n), value, 0),
+                     "GPUbitMemsetWithCuda");
+}
+
+hipError_t CPUbitAndWithCuda(BITS *host_a, BITS *host_b, unsigned int size) {
+  const size_t words = ref_words((int)size);
+  const size_t bytes = words * sizeof(BITS);
+  BITS *da = nullptr, *db = nullptr;
+  hipError_t e = hipMalloc((void **)&da, bytes);
+  if (e == hipSuccess) e = hipMalloc((void **)&db, bytes);
+  if (e == hipSuccess) e = hipMemcpy(da, host_a, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(db, host_b, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = bitop(kAnd, da, db, nullptr, words, 0, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipMemcpy(host_a, da, bytes, hipMemcpyDeviceToHost);
+  if (da) (void)hipFree(da);
+  if (db) (void)hipFree(db);
+  return set_error(e, "CPUbitAndWithCuda");
+}
+
+hipError_t refineWithCuda(CODE *bitmap, unsigned int size) {
+  const size_t bytes = (size_t)size * sizeof(BITS);
+  if (size == 0) return hipSuccess;
+  BITS *d = nullptr;
+  hipError_t e = hipMalloc((void **)&d, bytes);
+  if (e == hipSuccess) e = hipMemcpy(d, bitmap, bytes, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = flip_rows(d, size, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipMemcpy(bitmap, d, bytes, hipMemcpyDeviceToHost);
+  if (d) (void)hipFree(d);
+  return set_error(e, "refineWithCuda");
+}
+
+// ---------------------------------------------------------------------- predicates
+
+int rts_predicate_interval(enum OPERATOR op, CODE t1, CODE t2, CODE *lo, CODE *hi) {
+  CODE a = 0, b = 0xFFFFFFFFu;
+  switch (op) {
+    case LT:
+      if (t1 == 0) return 0;
+      b = t1 - 1;
+      break;
+    case LE: b = t1; break;
+    case GT:
+      if (t1 == 0xFFFFFFFFu) return 0;
+      a = t1 + 1;
+      break;
+    case GE: a = t1; break;
+    case EQ: a = b = t1; break;
+    case BT: {
+      CODE x = t1 < t2 ? t1 : t2, y = t1 < t2 ? t2 : t1;
+      if ((uint64_t)y <= (uint64_t)x + 1) return 0;
+      a = x + 1;
+      b = y - 1;
+      break;
+    }
+    default: return 0;
+  }
+  *lo = a;
+  *hi = b;
+  return 1;
+}
+
+// ------------------------------------------------------------------------ scans
+
+hipError_t rts_scan_range(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
+                          unsigned long long *count, hipStream_t stream) {
+  const CODE *cols[1] = {column};
+  return set_error(scan_conjunction(cols, &lo, &hi, 1, n, out, count, kStoreSet, stream),
+                   "rts_scan_range");
+}
+
+hipError_t rts_scan_range_xor(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
+                              hipStream_t stream) {
+  const CODE *cols[1] = {column};
+  return set_error(scan_conjunction(cols, &lo, &hi, 1, n, out, nullptr, kStoreXor, stream),
+                   "rts_scan_range_xor");
+}
+
+hipError_t rts_conjunction_scan(const CODE *const *columns, const CODE *lo, const CODE *hi,
+                                int ncols, uint64_t n, BITS *out, unsigned long long *count,
+                                hipStream_t stream) {
+  return set_error(scan_conjunction(columns, lo, hi, ncols, n, out, count, kStoreSet, stream),
+                   "rts_conjunction_scan");
+}
+
+hipError_t rts_bitmap_and_popcount(const BITS *const *bitmaps, int nbitmaps, uint64_t n,
+                                   BITS *out, unsigned long long *count, hipStream_t stream) {
+  return set_error(bitmap_and_popcount(bitmaps, nbitmaps, n, out, count, stream),
+                   "rts_bitmap_and_popcount");
+}
+
+hipError_t rts_bitmap_popcount(const BITS *bitmap, uint64_t n, unsigned long long *count,
+                               hipStream_t stream) {
+  if (!count) return set_error(hipErrorInvalidValue, "rts_bitmap_popcount");
+  return set_error(bitmap_and_popcount(&bitmap, 1, n, nullptr, count, stream),
+                   "rts_bitmap_popcount");
+}
+
+// raw_scan (bindex.cpp:1851-1892) evaluated on the GPU: host column, host bitmap, every
+// hit XOR-toggled into the bitmap.  With compare_bitmap given, the reference only reads
+// (and its check is a no-op, bindex.cpp:1880-1886), so nothing is written.
+void raw_scan(struct BinDex *bindex, BITS *bitmap, CODE t1, CODE t2, enum OPERATOR op,
+              CODE *raw_data, BITS *compare_bitmap) {
+  if (!bindex || !bitmap || !raw_data || compare_bitmap) return;
+  const int64_t n = bindex->length;
+  if (n <= 0) return;
+  if (op == BT && t1 >= t2) return;  // a < x < b with b <= a: no row (no swap here)
+  CODE lo, hi;
+  if (!rts_predicate_interval(op, t1, t2, &lo, &hi)) return;
+  const size_t words = words_for_rows((uint64_t)n);
+  CODE *dcol = nullptr;
+  BITS *dbm = nullptr;
+  hipError_t e = hipMalloc((void **)&dcol, (size_t)n * sizeof(CODE));
+  if (e == hipSuccess) e = hipMalloc((void **)&dbm, words * sizeof(BITS));
+  if (e == hipSuccess) e = hipMemcpy(dcol, raw_data, (size_t)n * sizeof(CODE), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(dbm, bitmap, words * sizeof(BITS), hipMemcpyHostToDevice);
+  const CODE *cols[1] = {dcol};
+  if (e == hipSuccess) e = scan_conjunction(cols, &lo, &hi, 1, (uint64_t)n, dbm, nullptr, kStoreXor, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipMemcpy(bitmap, dbm, words * sizeof(BITS), hipMemcpyDeviceToHost);
+  if (dcol) (void)hipFree(dcol);
+  if (dbm) (void)hipFree(dbm);
+  (void)set_error(e, "raw_scan");
+}
+
+// ------------------------------------------------------------------- helpers
+
+hipError_t rts_malloc(void **ptr, size_t bytes) { return set_error(hipMalloc(ptr, bytes), "rts_malloc"); }
+hipError_t rts_free(void *ptr) { return set_error(hipFree(ptr), "rts_free"); }
+hipError_t rts_memcpy_htod(void *dst, const void *src, size_t bytes) {
+  return set_error(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice), "rts_memcpy_htod");
+}
+hipError_t rts_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
+  return set_error(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "rts_memcpy_dtoh");
+}
+hipError_t rts_memcpy_dtod(void *dst, const void *src, size_t bytes) {
+  return set_error(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "rts_memcpy_dtod");
+}
+hipError_t rts_memset(void *dst, int v, size_t bytes) {
+  return set_error(hipMemset(dst, v, bytes), "rts_memset");
+}
+hipError_t rts_device_synchronize(void) {
+  return set_error(hipDeviceSynchronize(), "rts_device_synchronize");
+}
+hipError_t rts_stream_create(hipStream_t *s) {
+  return set_error(hipStreamCreateWithFlags(s, hipStreamNonBlocking), "rts_stream_create");
+}
+hipError_t rts_stream_destroy(hipStream_t s) {
+  return set_error(hipStreamDestroy(s), "rts_stream_destroy");
+}
+hipError_t rts_stream_synchronize(hipStream_t s) {
+  return set_error(hipStreamSynchronize(s), "rts_stream_synchronize");
+}
+hipError_t rts_event_create(hipEvent_t *e) { return set_error(hipEventCreate(e), "rts_event_create"); }
+hipError_t rts_event_destroy(hipEvent_t e) { return set_error(hipEventDestroy(e), "rts_event_destroy"); }
+hipError_t rts_event_record(hipEvent_t e, hipStream_t s) {
+  return set_error(hipEventRecord(e, s), "rts_event_record");
+}
+hipError_t rts_event_synchronize(hipEvent_t e) {
+  return set_error(hipEventSynchronize(e), "rts_event_synchronize");
+}
+hipError_t rts_event_elapsed_ms(float *ms, hipEvent_t a, hipEvent_t b) {
+  return set_error(hipEventElapsedTime(ms, a, b), "rts_event_elapsed_ms");
+}
+
+int rts_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+hipError_t rts_device_info(char *name, size_t name_len, int *cu_count, size_t *total_mem) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return set_error(e, "rts_device_info");
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) return set_error(e, "rts_device_info");
+  if (name && name_len) {
+    strncpy(name, prop.gcnArchName, name_len - 1);
+    name[name_len - 1] = 0;
+  }
+  if (cu_count) *cu_count = prop.multiProcessorCount;
+  if (total_mem) *total_mem = prop.totalGlobalMem;
+  return hipSuccess;
+}
+
+const char *rts_last_error(void) { return g_last_error.c_str(); }
+
+}  // extern "C"

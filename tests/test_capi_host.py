@@ -1,0 +1,124 @@
+"""Host-side checks of the C-ABI library (no GPU needed): it loads, exports every symbol the
+headers in include/ declare, and its host logic (predicate normalisation, seeded
+generators) is right."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+import pytest
+
+import rtscan_amd as R
+from _util import MAXU, closed_form
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for path in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        src = open(path).read()
+        src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)
+        src = re.sub(r"//[^\n]*", " ", src)
+        src = "\n".join(l for l in src.splitlines()
+                        if not l.strip().startswith("#") and l.strip() not in ('extern "C" {', "}"))
+        # drop struct / enum bodies
+        while True:
+            new = re.sub(r"\{[^{}]*\}", " ", src)
+            if new == src:
+                break
+            src = new
+        for stmt in src.split(";"):
+            stmt = " ".join(stmt.split())
+            if "(" not in stmt or stmt.startswith("typedef") or stmt.startswith("extern"):
+                continue
+            m = re.search(r"(\w+)\s*\(", stmt)
+            if m:
+                names.add(m.group(1))
+    return names
+
+
+def test_headers_declare_the_reference_surface():
+    names = declared_functions()
+    for ref_name in ["GPUbitAndWithCuda", "GPUbitOrWithCuda", "GPUbitCopyWithCuda",
+                     "GPUbitCopyNegationWithCuda", "GPUbitCopySIMDWithCuda",
+                     "GPUbitMemsetWithCuda", "CPUbitAndWithCuda", "refineWithCuda", "raw_scan"]:
+        assert ref_name in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(R.LIB_PATH)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_python_front_end_covers_every_declared_symbol():
+    missing = sorted(declared_functions() - set(R.SIGNATURES))
+    assert not missing, missing
+    R.lib()  # binds every signature
+
+
+EDGES = [0, 1, 2, 1000, 0x7FFFFFFF, 0x80000000, MAXU - 2, MAXU - 1, MAXU]
+
+
+@pytest.mark.parametrize("op", [R.EQ, R.LT, R.GT, R.LE, R.GE, R.BT])
+def test_predicate_interval_equals_reference_predicate(op):
+    """The closed interval selects exactly the rows raw_scan's predicate selects
+    (bindex.cpp:1857-1874), including at 0 and 2^32-1 and for swapped bt bounds."""
+    xs = np.array(sorted(set(EDGES + [e + d for e in EDGES for d in (-1, 1) if 0 <= e + d <= MAXU])),
+                  np.uint32)
+    for t1 in EDGES:
+        for t2 in (EDGES if op == R.BT else [0]):
+            a, b = (min(t1, t2), max(t1, t2)) if op == R.BT else (t1, t2)
+            want = closed_form(xs, op, a, b)
+            iv = R.predicate_interval(op, t1, t2)
+            if iv is None:
+                assert not want.any()
+                # empty means no uint32 at all qualifies
+                if op == R.LT:
+                    assert t1 == 0
+                elif op == R.GT:
+                    assert t1 == MAXU
+                elif op == R.BT:
+                    assert max(t1, t2) <= min(t1, t2) + 1
+                else:
+                    raise AssertionError("le/ge/eq are never empty")
+            else:
+                lo, hi = iv
+                assert lo <= hi
+                assert np.array_equal((xs >= lo) & (xs <= hi), want), (op, t1, t2)
+
+
+def test_generator_is_deterministic_across_thread_counts():
+    for dist, p0, p1 in ((R.DIST_UNIFORM, 0, 0), (R.DIST_NORMAL, 2.0 ** 31, 2.0 ** 28),
+                         (R.DIST_ZIPF, 1.1, 2.0 ** 20)):
+        a = R.generate_column(dist, 42, 3, 100003, p0, p1, threads=1)
+        b = R.generate_column(dist, 42, 3, 100003, p0, p1, threads=7)
+        assert np.array_equal(a, b)
+        c = R.generate_column(dist, 43, 3, 1000, p0, p1)
+        assert not np.array_equal(a[:1000], c)
+
+
+def test_generator_distributions():
+    n = 400000
+    u = R.generate_column(R.DIST_UNIFORM, 1, 0, n)
+    assert abs(u.astype(np.float64).mean() / 2 ** 32 - 0.5) < 0.005
+    assert abs((u < 2 ** 31).mean() - 0.5) < 0.005
+    g = R.generate_column(R.DIST_NORMAL, 1, 1, n, 2.0 ** 31, 2.0 ** 28).astype(np.float64)
+    assert abs(g.mean() - 2 ** 31) < 0.01 * 2 ** 28
+    assert abs(g.std() / 2 ** 28 - 1) < 0.01
+    for s in (1.1, 1.5):
+        z = R.generate_column(R.DIST_ZIPF, 1, 2, n, s, 2.0 ** 20)
+        vals, cnt = np.unique(z, return_counts=True)
+        assert len(vals) <= 2 ** 20
+        h = np.sum(np.arange(1, 2 ** 20 + 1, dtype=np.float64) ** -s)
+        p_top = 1.0 / h
+        assert abs(cnt.max() / n - p_top) < 0.01, (s, cnt.max() / n, p_top)
+
+
+def test_generator_rejects_bad_parameters():
+    with pytest.raises(ValueError):
+        R.generate_column(R.DIST_ZIPF, 1, 0, 10, -1.0, 100)
+    with pytest.raises(ValueError):
+        R.generate_column(7, 1, 0, 10)
