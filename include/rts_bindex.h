@@ -49,7 +49,68 @@ typedef struct BinDex {
   struct rts_bindex_dev *dev;    /* engine-owned device index */
 } BinDex;
 
+/* ------------------------------------------------ reference surface (bindex.h:145-183) */
+
+/* copy_filter_vector_in_GPU (bindex.h:145, bindex_cuda.cpp:486-504): dev_bitmap =
+ * fv[k] (negation: ~fv[k]) over bits_num_needed(length) words; k < 0 gives all zeros and
+ * k >= K-1 all ones.  Synchronous. */
+void copy_filter_vector_in_GPU(BinDex *bindex, BITS *dev_bitmap, int k,
+                               bool negation RTS_DEFAULT(false));
+
+/* GPURefineAreaWithCuda (bindex.h:180, cuda_and.cu:510-640).  predicate: host array of
+ * column_num (lo, hi) pairs, exclusive bounds (lo < x < hi).  Refines the blocks of the
+ * selected column's boundary area: blocks [0, blockIdx] set the bit of every row inside all
+ * bounds, or with inverse blocks [blockIdx, blockNum) clear it.  The reference launches one
+ * kernel per 3276-row block and leaks a device buffer per call; this is one launch over the
+ * contiguous slot range, no allocation.  Synchronous. */
+hipError_t GPURefineAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CODE *predicate,
+                                 int selected_id, int column_num RTS_DEFAULT(3),
+                                 bool inverse RTS_DEFAULT(false));
+/* GPURefineEqAreaWithCuda (bindex.h:181, cuda_and.cu:642-812): the same test over the slots
+ * from the block of the selected column's smaller bound to the block of its larger bound. */
+hipError_t GPURefineEqAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CODE *predicate,
+                                   int selected_id, int column_num RTS_DEFAULT(3),
+                                   bool inverse RTS_DEFAULT(false));
+/* in_which_area (bindex.h:182, bindex.cpp:1138-1151) */
+int in_which_area(BinDex *bindex, CODE compare);
+/* in_which_block (bindex.h:183, bindex.cpp:1153-1176) */
+int in_which_block(Area *area, CODE compare);
+
+/* ------------------------------------------------------------------ additive BinDex API */
+
+/* Build the index of a device column (init_bindex, bindex.cpp:599-671) on the GPU.  The
+ * column is referenced, not copied (BinDex::rawDataInGPU), and must outlive the index.
+ * n >= RTS_VAREA_N.  Synchronous. */
+hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, hipStream_t stream);
+/* As rts_bindex_build for a host column, uploaded into a library-owned device copy. */
+hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n);
+/* Release the device index (and the library-owned column copy, if any). */
+void rts_bindex_free(BinDex *bindex);
+
+/* Kernel (b) fused with kernel (c): out = AND over ncols indexed columns of
+ * (lo[c] <= x_c <= hi[c]) evaluated from filter vectors and boundary-bin refine lists,
+ * popcount into `count` (device, may be NULL).  All columns have the same length; lo and
+ * hi are host arrays.  Asynchronous on `stream`. */
+hipError_t rts_bindex_query(const BinDex *const *bindexes, const CODE *lo, const CODE *hi,
+                            int ncols, BITS *out, unsigned long long *count, hipStream_t stream);
+
+/* Host copies of the value-sorted column and its row positions (ties in row order). */
+hipError_t rts_bindex_export_sorted(const BinDex *bindex, CODE *host_sorted_val,
+                                    int32_t *host_sorted_pos);
+/* Number of refine-list entries (rows strictly inside a bin), or -1. */
+long long rts_bindex_refine_entries(const BinDex *bindex);
+/* Device bytes held by the index (excluding the column), or -1. */
+long long rts_bindex_device_bytes(const BinDex *bindex);
+
 #ifdef __cplusplus
+}
+#endif
+
+#ifdef __cplusplus
+/* init_bindex_in_GPU (bindex_cuda.cpp:164-282): the reference driver's build call, with its
+ * signature.  base_data grows by the device bytes of the uploaded column, as there. */
+inline void init_bindex_in_GPU(BinDex *bindex, CODE *data, POSTYPE n, size_t &base_data) {
+  if (rts_bindex_build_host(bindex, data, n) == hipSuccess) base_data += (size_t)n * sizeof(CODE);
 }
 #endif
 #endif /* RTS_BINDEX_H_ */

@@ -26,6 +26,14 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* Default arguments of the reference's C++ declarations (bindex.h:145-183), kept for C++
+ * callers; C callers pass every argument. */
+#ifdef __cplusplus
+#define RTS_DEFAULT(value) = value
+#else
+#define RTS_DEFAULT(value)
+#endif
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -78,7 +86,7 @@ hipError_t refineWithCuda(CODE *bitmap, unsigned int size);
  * is only read for its `length`; see rts_bindex.h for the struct. */
 struct BinDex;
 void raw_scan(struct BinDex *bindex, BITS *bitmap, CODE target1, CODE target2,
-              enum OPERATOR OP, CODE *raw_data, BITS *compare_bitmap);
+              enum OPERATOR OP, CODE *raw_data, BITS *compare_bitmap RTS_DEFAULT(NULL));
 
 /* ---------------------------------------------------------- additive: device helpers */
 

@@ -87,6 +87,9 @@ def lib():
                                        ctypes.c_uint32]
         L.orc_bindex_scan.argtypes = [ctypes.POINTER(_OrcBindex), _u32p, ctypes.c_int,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        for f in ("orc_refine_area", "orc_refine_eq_area"):
+            getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _u32p,
+                                      ctypes.c_int, ctypes.c_int, _u32p]
         _lib = L
     return _lib
 
@@ -298,3 +301,16 @@ class _RefBindex:
         out = np.zeros(self.words, np.uint32)
         self.o.L.ref_bindex_scan(self.h, op, t1 & 0xFFFFFFFF, t2 & 0xFFFFFFFF, out)
         return out
+
+
+def refine_area(bindexes, columns, predicate, sel, inverse, bitmap, eq=False):
+    """GPURefineAreaWithCuda / GPURefineEqAreaWithCuda restated (cuda_and.cu:510-812) over
+    oracle Bindex objects; updates `bitmap` in place.  Returns False where the reference
+    would read out of bounds."""
+    k = len(bindexes)
+    bx = (ctypes.POINTER(_OrcBindex) * k)(*[ctypes.pointer(b.s) for b in bindexes])
+    cols = [np.ascontiguousarray(c, np.uint32) for c in columns]
+    rp = (ctypes.c_void_p * k)(*[c.ctypes.data for c in cols])
+    pred = np.ascontiguousarray(predicate, np.uint32)
+    f = lib().orc_refine_eq_area if eq else lib().orc_refine_area
+    return f(bx, rp, k, pred, sel, int(bool(inverse)), bitmap) == 0

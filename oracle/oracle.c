@@ -486,3 +486,87 @@ int orc_bindex_scan(const orc_bindex *bx, uint32_t *result, int op, uint32_t t1,
     default: return -1;
   }
 }
+
+/* ------------------------------------------------------- GPU refine-area surface */
+
+/* refineBlock, cuda_and.cu:74-99: the selected column is tested first, then the others;
+ * every bound is exclusive. */
+static void refine_block_rows(const orc_bindex *sb, const uint32_t *const *raws, int ncols,
+                              const uint32_t *pred, int sel, int inverse, int s0, int s1,
+                              uint32_t *bitmap) {
+  for (int s = s0; s < s1; s++) {
+    int p = sb->sorted_pos[s];
+    uint32_t x = raws[sel][p];
+    if (x <= pred[2 * sel] || x >= pred[2 * sel + 1]) continue;
+    int j;
+    for (j = 0; j < ncols; j++) {
+      if (j == sel) continue;
+      uint32_t y = raws[j][p];
+      if (y <= pred[2 * j] || y >= pred[2 * j + 1]) break;
+    }
+    if (j < ncols) continue;
+    if (inverse)
+      bitmap[p / BITSWIDTH] &= ~row_bit(p);
+    else
+      bitmap[p / BITSWIDTH] |= row_bit(p);
+  }
+}
+
+/* GPURefineAreaWithCuda, cuda_and.cu:510-640 (block choice cuda_and.cu:546-555, block
+ * ranges cuda_and.cu:579-628) */
+int orc_refine_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
+                    const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap) {
+  const orc_bindex *b = bxs[sel];
+  uint32_t compare = predicate[2 * sel + 1];
+  int area = orc_in_which_area(b, compare);
+  if (area < 0) return -1;
+  if (b->start_values[area] == compare) {
+    compare = predicate[2 * sel];
+    area = orc_in_which_area(b, compare);
+    if (area < 0) return -1;
+  }
+  int block = orc_in_which_block(b, area, compare);
+  if (block < 0) return -1;
+  int nb = orc_area_block_num(b, area);
+  int b0 = inverse ? block : 0, b1 = inverse ? nb - 1 : block;
+  if (b1 < b0) return 0;
+  int s0 = block_first(b, area, b0), s1 = block_first(b, area, b1) + block_len(b, area, b1);
+  refine_block_rows(b, raws, ncols, predicate, sel, inverse, s0, s1, bitmap);
+  return 0;
+}
+
+/* GPURefineEqAreaWithCuda, cuda_and.cu:642-812 */
+int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
+                       const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap) {
+  const orc_bindex *b = bxs[sel];
+  uint32_t left = predicate[2 * sel], right = predicate[2 * sel + 1];
+  if (left > right) {
+    uint32_t t = left;
+    left = right;
+    right = t;
+  }
+  int al = orc_in_which_area(b, left), ar = orc_in_which_area(b, right);
+  if (ar < 0) return -1;
+  int br = orc_in_which_block(b, ar, right);
+  if (br < 0) return -1;
+  if (al == ar) { /* cuda_and.cu:701-721 */
+    int bl = orc_in_which_block(b, al, left);
+    for (int j = bl; j <= br; j++)
+      refine_block_rows(b, raws, ncols, predicate, sel, inverse, block_first(b, al, j),
+                        block_first(b, al, j) + block_len(b, al, j), bitmap);
+    return 0;
+  }
+  if (al >= 0) { /* cuda_and.cu:725-750 */
+    int bl = orc_in_which_block(b, al, left);
+    for (int j = orc_area_block_num(b, al) - 1; j >= bl; j--)
+      refine_block_rows(b, raws, ncols, predicate, sel, inverse, block_first(b, al, j),
+                        block_first(b, al, j) + block_len(b, al, j), bitmap);
+  }
+  for (int i = al + 1; i <= ar - 1; i++) /* cuda_and.cu:751-776 */
+    refine_block_rows(b, raws, ncols, predicate, sel, inverse, b->area_start[i],
+                      b->area_start[i] + b->area_len[i], bitmap);
+  for (int j = 0; j <= br; j++) /* cuda_and.cu:777-799 */
+    refine_block_rows(b, raws, ncols, predicate, sel, inverse, block_first(b, ar, j),
+                      block_first(b, ar, j) + block_len(b, ar, j), bitmap);
+  return 0;
+}

@@ -81,6 +81,16 @@ int orc_on_which_pos(const orc_bindex *bx, int area, int block, uint32_t c);
 int orc_bindex_scan(const orc_bindex *bx, uint32_t *result, int op, uint32_t t1, uint32_t t2,
                     int threads);
 
+/* GPURefineAreaWithCuda / GPURefineEqAreaWithCuda (cuda_and.cu:510-812) with refineBlock
+ * (cuda_and.cu:74-99): set (inverse: clear) the bit of every row in the chosen blocks of the
+ * selected column's area(s) whose value in every column lies strictly inside its
+ * (lo, hi) pair of `predicate`.  `bxs[c]` indexes `raws[c]`.  Returns -1 where the
+ * reference would read out of bounds (bound below the data). */
+int orc_refine_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
+                    const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap);
+int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
+                       const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap);
+
 #ifdef __cplusplus
 }
 #endif

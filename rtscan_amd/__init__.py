@@ -66,6 +66,30 @@ SIGNATURES = {
     "rts_conjunction_scan": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
     "rts_bitmap_and_popcount": (_err, [_vp, _int, _u64, _vp, _vp, _vp]),
     "rts_bitmap_popcount": (_err, [_vp, _u64, _vp, _vp]),
+    # BinDex surface
+    "copy_filter_vector_in_GPU": (None, [_vp, _vp, _int, ctypes.c_bool]),
+    "GPURefineAreaWithCuda": (_err, [_vp, _vp, _vp, _int, _int, ctypes.c_bool]),
+    "GPURefineEqAreaWithCuda": (_err, [_vp, _vp, _vp, _int, _int, ctypes.c_bool]),
+    "in_which_area": (_int, [_vp, _u32]),
+    "in_which_block": (_int, [_vp, _u32]),
+    "rts_bindex_build": (_err, [_vp, _vp, _int, _vp]),
+    "rts_bindex_build_host": (_err, [_vp, _vp, _int]),
+    "rts_bindex_free": (None, [_vp]),
+    "rts_bindex_query": (_err, [_vp, _vp, _vp, _int, _vp, _vp, _vp]),
+    "rts_bindex_export_sorted": (_err, [_vp, _vp, _vp]),
+    "rts_bindex_refine_entries": (ctypes.c_longlong, [_vp]),
+    "rts_bindex_device_bytes": (ctypes.c_longlong, [_vp]),
+    # query / data files
+    "rts_query_file_parse": (_int, [ctypes.c_char_p, ctypes.c_size_t, _int, ctypes.POINTER(_vp),
+                                    ctypes.POINTER(_int)]),
+    "rts_query_file_load": (_int, [ctypes.c_char_p, _int, ctypes.POINTER(_vp),
+                                   ctypes.POINTER(_int)]),
+    "rts_query_file_count": (_int, [_vp]),
+    "rts_query_file_columns": (_int, [_vp]),
+    "rts_query_file_get": (_vp, [_vp, _int, _int]),
+    "rts_query_file_free": (None, [_vp]),
+    "rts_data_file_read": (_int, [ctypes.c_char_p, _int, ctypes.c_int64, _vp]),
+    "rts_data_file_write": (_int, [ctypes.c_char_p, _int, ctypes.c_int64, _vp]),
     # generators
     "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
                                    _vp, _int]),
@@ -265,3 +289,120 @@ def generate_column(dist, seed, column_id, n, p0=0.0, p1=0.0, threads=None):
     if rc:
         raise ValueError("rts_generate_column: bad parameters")
     return out
+
+
+class BinDexIndex:
+    """A BinDex (include/rts_bindex.h) built on the GPU over a device column."""
+
+    def __init__(self, column, n, stream=None):
+        from .bindex_struct import BinDex
+
+        self.column = column  # keep the device column alive
+        self.n = n
+        self.s = BinDex()
+        check(lib().rts_bindex_build(ctypes.byref(self.s), column.ptr, n, _sh(stream)),
+              "rts_bindex_build")
+        self.built = True
+
+    @property
+    def ptr(self):
+        return ctypes.addressof(self.s)
+
+    def free(self):
+        if getattr(self, "built", False):
+            lib().rts_bindex_free(ctypes.byref(self.s))
+            self.built = False
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def start_values(self):
+        return np.array(self.s.areaStartValues[:], np.uint32)
+
+    def area_counts(self):
+        return np.array(self.s.area_counts[:], np.int64)
+
+    def filter_vector(self, k):
+        W = (self.n + 31) // 32
+        out = np.empty(W, np.uint32)
+        check(lib().rts_memcpy_dtoh(out.ctypes.data, self.s.filterVectorsInGPU[k], 4 * W), "dtoh")
+        return out
+
+    def sorted(self):
+        v = np.empty(self.n, np.uint32)
+        p = np.empty(self.n, np.int32)
+        check(lib().rts_bindex_export_sorted(ctypes.byref(self.s), v.ctypes.data, p.ctypes.data),
+              "export_sorted")
+        return v, p
+
+    def refine_entries(self):
+        return int(lib().rts_bindex_refine_entries(ctypes.byref(self.s)))
+
+    def device_bytes(self):
+        return int(lib().rts_bindex_device_bytes(ctypes.byref(self.s)))
+
+
+def bindex_query(indexes, los, his, out, count=None, stream=None):
+    """Kernel (b)+(c): AND over indexed columns of lo_c <= x_c <= hi_c, plus popcount."""
+    pa = (_vp * len(indexes))(*[ix.ptr for ix in indexes])
+    lo = np.ascontiguousarray(los, np.uint32)
+    hi = np.ascontiguousarray(his, np.uint32)
+    check(lib().rts_bindex_query(pa, lo.ctypes.data, hi.ctypes.data, len(indexes), out.ptr,
+                                 count.ptr if count else None, _sh(stream)), "rts_bindex_query")
+
+
+class _ColumnPredicate(ctypes.Structure):
+    _fields_ = [("op", ctypes.c_int), ("ntargets", ctypes.c_int),
+                ("t1", ctypes.POINTER(ctypes.c_uint32)), ("t2", ctypes.POINTER(ctypes.c_uint32))]
+
+
+class QueryFileError(ValueError):
+    pass
+
+
+def parse_query_file(text=None, path=None, ncols=1):
+    """Parse the reference query-file format (rts_query.h) with the library's parser.
+    Returns a list of queries, each a list of (op, [t1...], [t2...] or None) per column."""
+    h = _vp()
+    line = _int()
+    if path is not None:
+        rc = lib().rts_query_file_load(path.encode(), ncols, ctypes.byref(h), ctypes.byref(line))
+    else:
+        data = text.encode() if isinstance(text, str) else bytes(text)
+        rc = lib().rts_query_file_parse(data, len(data), ncols, ctypes.byref(h), ctypes.byref(line))
+    if rc:
+        raise QueryFileError("query file error %d at line %d" % (rc, line.value))
+    try:
+        out = []
+        for q in range(lib().rts_query_file_count(h)):
+            cols = []
+            for c in range(ncols):
+                p = ctypes.cast(lib().rts_query_file_get(h, q, c),
+                                ctypes.POINTER(_ColumnPredicate)).contents
+                t1 = [p.t1[i] for i in range(p.ntargets)]
+                t2 = [p.t2[i] for i in range(p.ntargets)] if p.t2 else None
+                cols.append((p.op, t1, t2))
+            out.append(cols)
+        return out
+    finally:
+        lib().rts_query_file_free(h)
+
+
+def write_data_file(path, columns):
+    cols = [np.ascontiguousarray(c, np.uint32) for c in columns]
+    arr = (_vp * len(cols))(*[c.ctypes.data for c in cols])
+    rc = lib().rts_data_file_write(path.encode(), len(cols), cols[0].size, arr)
+    if rc:
+        raise OSError("rts_data_file_write failed: %d" % rc)
+
+
+def read_data_file(path, ncols, n):
+    cols = [np.empty(n, np.uint32) for _ in range(ncols)]
+    arr = (_vp * ncols)(*[c.ctypes.data for c in cols])
+    rc = lib().rts_data_file_read(path.encode(), ncols, n, arr)
+    if rc:
+        raise OSError("rts_data_file_read failed: %d" % rc)
+    return cols

@@ -1,0 +1,88 @@
+// bindex_dev.h -- the engine's device-resident BinDex index (behind BinDex::dev).
+//
+// Per column, K = 128 bins from the reference's equal-frequency area starts
+// (init_bindex, bindex.cpp:599-671):
+//   sorted_val / sorted_pos   the column sorted by value (stable: ties in row order), n each
+//   fvs                       K-1 filter vectors, fv[k] bit i = (x_i < start[k+1]),
+//                             fv_stride words apart (stride padded to 64 words)
+//   B lists                   for every bin k (value range [start[k], start[k+1])), the rows
+//                             with start[k] < x < start[k+1] in row order: bpos (row), bval
+//                             (value); bin k's entries are [boff[k], boff[k+1])
+//   bdir                      tile directory: bin k's first entry with row >= t * kTileRows
+//                             is bdir[k * (ntiles + 1) + t]
+// A bound v of a query (the set L(v) = {x < v}) is then one filter vector plus the rows of
+// one bin's B list with x >= v: L(v) = fv[t(v)] minus {B_t(v) rows with x >= v}, or
+// fv[t(v) - 1] exactly when v is a bin start.  Rows equal to a bin start never need a
+// refine, so a heavily duplicated value (Zipf) costs nothing at query time.
+#pragma once
+
+#include <stdint.h>
+
+#include <vector>
+
+#include "common.h"
+#include "rts_bindex.h"
+
+namespace rts {
+
+constexpr int kBins = RTS_VAREA_N;
+constexpr int kTileWords = 1024;              // bitmap words per query-kernel tile
+constexpr int kTileRows = kTileWords * 32;    // 32768 rows
+constexpr int kFvStrideAlign = 64;            // words
+
+}  // namespace rts
+
+struct rts_bindex_dev {
+  int64_t n = 0;
+  int64_t words = 0;
+  int64_t fv_stride = 0;
+  int64_t ntiles = 0;
+  int64_t nb = 0;  // total B-list entries
+  CODE *sorted_val = nullptr;
+  int32_t *sorted_pos = nullptr;
+  BITS *fvs = nullptr;
+  uint32_t *bpos = nullptr;
+  CODE *bval = nullptr;
+  uint32_t *boff = nullptr;  // device, kBins + 1
+  uint32_t *bdir = nullptr;  // device, kBins * (ntiles + 1)
+  CODE *owned_column = nullptr;
+  // host copies
+  CODE starts[rts::kBins];
+  int64_t area_start[rts::kBins];
+  int64_t area_len[rts::kBins];
+  std::vector<uint32_t> boff_host;
+  std::vector<Area> area_store;
+  std::vector<CODE> block_first, block_last;
+};
+
+namespace rts {
+
+// Bin of value v: the number of k in [1, K-1] with start[k] <= v.
+inline int bin_of(const CODE *starts, CODE v) {
+  int lo = 1, hi = kBins;  // first k in [1, K) with start[k] > v
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (starts[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  return lo - 1;
+}
+
+// One query bound L(v) = {x < v} as (base filter vector, refine list).
+struct BoundSpec {
+  int fv;        // -1: no rows, kBins - 1: all rows, else filter vector index
+  int list;      // -1: no refine, else bin whose B-list rows with x >= thr leave the set
+  CODE thr;
+};
+
+inline BoundSpec bound_spec(const rts_bindex_dev *d, CODE v, bool infinite) {
+  if (infinite) return BoundSpec{kBins - 1, -1, 0};  // v = 2^32: every row
+  if (v <= d->starts[0]) return BoundSpec{-1, -1, 0};  // v <= min: no row is < v
+  const int k = bin_of(d->starts, v);
+  if (v == d->starts[k]) return BoundSpec{k - 1, -1, 0};
+  return BoundSpec{k, d->boff_host[k + 1] > d->boff_host[k] ? k : -1, v};
+}
+
+}  // namespace rts

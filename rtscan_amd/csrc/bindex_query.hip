@@ -1,0 +1,428 @@
+// bindex_query.hip -- kernel (b): BinDex sieve + boundary refine, fused with the
+// multi-column AND and popcount (kernel c), plus the reference's BinDex call surface
+// (copy_filter_vector_in_GPU, GPURefineAreaWithCuda, GPURefineEqAreaWithCuda,
+// in_which_area, in_which_block).
+//
+// Query of one column for the closed range [a, b]: rows(a <= x <= b) = L(b+1) & ~L(a) with
+// L(v) = {x < v}.  Each L(v) is a filter vector with the rows of one bin's B list that
+// have x >= v removed (see bindex_dev.h).  A workgroup owns a tile of kTileWords output
+// words: it marks the tile's B-list rows to remove in LDS (LDS atomics, no global
+// atomics), then streams the filter-vector words once with 128-bit loads and writes each
+// output word once.  Traffic per column: two filter vectors and two bins' list slices.
+#include <string.h>
+
+#include <algorithm>
+
+#include "bindex_dev.h"
+
+namespace rts {
+
+hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s);
+void bindex_free(BinDex *bx);
+hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
+                 hipStream_t s);
+
+enum : uint32_t { kBaseZero = 0, kBaseOnes = 1, kBaseFv = 2 };
+
+struct BoundDev {
+  const BITS *fv;       // kBaseFv
+  const uint32_t *dir;  // tile directory row of the bin (nullptr: no refine)
+  const uint32_t *bpos;
+  const CODE *bval;
+  CODE thr;
+  uint32_t base;
+};
+
+template <int C>
+struct QueryArgs {
+  BoundDev lo[C];  // L(a)
+  BoundDev hi[C];  // L(b + 1)
+};
+
+__device__ __forceinline__ void mark_removed(const BoundDev &b, int64_t t, uint32_t *tog) {
+  if (!b.dir) return;
+  const uint32_t e0 = b.dir[t], e1 = b.dir[t + 1];
+  const uint32_t row0 = (uint32_t)(t * kTileRows);
+  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
+    const uint32_t p = b.bpos[e];
+    if (b.bval[e] >= b.thr) atomicOr(&tog[(p - row0) >> 5], 0x80000000u >> (p & 31));
+  }
+}
+
+__device__ __forceinline__ uint4 base_words(const BoundDev &b, int64_t w, int64_t words) {
+  if (b.base == kBaseZero) return make_uint4(0, 0, 0, 0);
+  if (b.base == kBaseOnes) return make_uint4(~0u, ~0u, ~0u, ~0u);
+  if (w + 3 < words) return *reinterpret_cast<const uint4 *>(b.fv + w);
+  uint4 v = make_uint4(0, 0, 0, 0);
+  v.x = b.fv[w];
+  if (w + 1 < words) v.y = b.fv[w + 1];
+  if (w + 2 < words) v.z = b.fv[w + 2];
+  return v;
+}
+
+template <int C, int MODE, bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t n, int64_t words,
+                                                         int64_t ntiles, BITS *out, bool out_vec,
+                                                         CountSink sink) {
+  __shared__ uint32_t tog[C][2][kTileWords];
+  constexpr int kPer = kTileWords / kBlock;  // 4 words per thread
+  static_assert(kPer == 4, "one uint4 of output words per thread");
+  const uint32_t last_mask = tail_mask((uint64_t)n);
+  uint32_t pc = 0;
+  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    for (int i = threadIdx.x; i < C * 2 * kTileWords; i += kBlock) (&tog[0][0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      mark_removed(q.lo[c], t, tog[c][0]);
+      mark_removed(q.hi[c], t, tog[c][1]);
+    }
+    __syncthreads();
+    const int lw = threadIdx.x * kPer;
+    const int64_t w = t * kTileWords + lw;
+    if (w < words) {
+      uint4 acc = make_uint4(~0u, ~0u, ~0u, ~0u);
+#pragma unroll
+      for (int c = 0; c < C; c++) {
+        uint4 lo = base_words(q.lo[c], w, words), hi = base_words(q.hi[c], w, words);
+        const uint32_t *tl = &tog[c][0][lw], *th = &tog[c][1][lw];
+        acc.x &= (hi.x ^ th[0]) & ~(lo.x ^ tl[0]);
+        acc.y &= (hi.y ^ th[1]) & ~(lo.y ^ tl[1]);
+        acc.z &= (hi.z ^ th[2]) & ~(lo.z ^ tl[2]);
+        acc.w &= (hi.w ^ th[3]) & ~(lo.w ^ tl[3]);
+      }
+      uint32_t v[4] = {acc.x, acc.y, acc.z, acc.w};
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        if (w + j == words - 1) v[j] &= last_mask;
+        if (w + j >= words) v[j] = 0;
+      }
+      if (MODE == kStoreAnd) {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (w + j < words) v[j] &= out[w + j];
+      }
+      if (out_vec && w + 3 < words) {
+        *reinterpret_cast<uint4 *>(out + w) = make_uint4(v[0], v[1], v[2], v[3]);
+      } else {
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+          if (w + j < words) out[w + j] = v[j];
+      }
+      if (COUNT) pc += __popc(v[0]) + __popc(v[1]) + __popc(v[2]) + __popc(v[3]);
+    }
+    __syncthreads();
+  }
+  if (COUNT) count_finish(pc, sink);
+}
+
+static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
+  BoundDev b{};
+  if (s.fv < 0) {
+    b.base = kBaseZero;
+  } else if (s.fv >= kBins - 1) {
+    b.base = kBaseOnes;
+  } else {
+    b.base = kBaseFv;
+    b.fv = d->fvs + (size_t)s.fv * d->fv_stride;
+  }
+  if (s.list >= 0) {
+    b.dir = d->bdir + (size_t)s.list * (d->ntiles + 1);
+    b.bpos = d->bpos;
+    b.bval = d->bval;
+    b.thr = s.thr;
+  }
+  return b;
+}
+
+template <int C, int MODE>
+static hipError_t launch_query(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
+                               unsigned long long *count, hipStream_t s) {
+  const bool vec = ((uintptr_t)out & 15) == 0;
+  const unsigned grid = (unsigned)std::min<int64_t>(d->ntiles, (int64_t)device_cu_count() * 32);
+  if (count) {
+    CountSink sink{count, stream_count_scratch(s)};
+    if (!sink.scr) return hipErrorOutOfMemory;
+    hipLaunchKernelGGL((k_bindex_query<C, MODE, true>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
+                       d->words, d->ntiles, out, vec, sink);
+  } else {
+    hipLaunchKernelGGL((k_bindex_query<C, MODE, false>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
+                       d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr});
+  }
+  return hipGetLastError();
+}
+
+template <int C>
+static hipError_t query_group(const BinDex *const *bxs, const CODE *lo, const CODE *hi, int mode,
+                              BITS *out, unsigned long long *count, hipStream_t s) {
+  QueryArgs<C> q;
+  for (int c = 0; c < C; c++) {
+    const rts_bindex_dev *d = bxs[c]->dev;
+    const BoundSpec sl = lo[c] == 0 ? BoundSpec{-1, -1, 0} : bound_spec(d, lo[c], false);
+    const BoundSpec sh = bound_spec(d, hi[c] + 1u, hi[c] == 0xFFFFFFFFu);
+    q.lo[c] = make_bound(d, sl);
+    q.hi[c] = make_bound(d, sh);
+  }
+  if (mode == kStoreAnd) return launch_query<C, kStoreAnd>(q, bxs[0]->dev, out, count, s);
+  return launch_query<C, kStoreSet>(q, bxs[0]->dev, out, count, s);
+}
+
+__global__ void k_zero_count(unsigned long long *p) { *p = 0ull; }
+
+hipError_t bindex_query(const BinDex *const *bxs, const CODE *lo, const CODE *hi, int ncols,
+                        BITS *out, unsigned long long *count, hipStream_t s) {
+  if (!bxs || !lo || !hi || ncols < 1 || !out) return hipErrorInvalidValue;
+  const rts_bindex_dev *d0 = bxs[0] ? bxs[0]->dev : nullptr;
+  if (!d0) return hipErrorInvalidValue;
+  for (int c = 0; c < ncols; c++) {
+    if (!bxs[c] || !bxs[c]->dev || bxs[c]->dev->n != d0->n) return hipErrorInvalidValue;
+  }
+  for (int c = 0; c < ncols; c++) {
+    if (lo[c] > hi[c]) {  // empty predicate: no row qualifies
+      if (count) {
+        hipLaunchKernelGGL(k_zero_count, dim3(1), dim3(1), 0, s, count);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+      }
+      return bitop(5 /* fill */, out, nullptr, nullptr, d0->words, 0u, s);
+    }
+  }
+  int done = 0;
+  while (done < ncols) {
+    const int g = std::min(4, ncols - done);
+    const int mode = done == 0 ? kStoreSet : kStoreAnd;
+    unsigned long long *cnt = done + g == ncols ? count : nullptr;
+    hipError_t e;
+    switch (g) {
+      case 1: e = query_group<1>(bxs + done, lo + done, hi + done, mode, out, cnt, s); break;
+      case 2: e = query_group<2>(bxs + done, lo + done, hi + done, mode, out, cnt, s); break;
+      case 3: e = query_group<3>(bxs + done, lo + done, hi + done, mode, out, cnt, s); break;
+      default: e = query_group<4>(bxs + done, lo + done, hi + done, mode, out, cnt, s); break;
+    }
+    if (e != hipSuccess) return e;
+    done += g;
+  }
+  return hipSuccess;
+}
+
+// ------------------------------------------------- refineBlock (cuda_and.cu:74-99)
+
+constexpr int kMaxRefineCols = 16;
+struct RefineCols {
+  const CODE *raw[kMaxRefineCols];
+  CODE lo[kMaxRefineCols];  // exclusive bounds: lo < x < hi
+  CODE hi[kMaxRefineCols];
+};
+
+// For every sorted slot in [s0, s1) of the selected column: if every column's value at the
+// slot's row lies strictly inside its bounds, set (or, inverse, clear) the row's bit.
+__global__ __launch_bounds__(kBlock) void k_refine_slots(RefineCols rc, int ncols,
+                                                         const int32_t *sorted_pos, int64_t s0,
+                                                         int64_t s1, BITS *bitmap, bool inverse) {
+  for (int64_t s = s0 + (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < s1;
+       s += (int64_t)gridDim.x * blockDim.x) {
+    const int32_t p = sorted_pos[s];
+    bool ok = true;
+    for (int j = 0; j < ncols && ok; j++) {
+      const CODE x = rc.raw[j][p];
+      ok = rc.lo[j] < x && x < rc.hi[j];
+    }
+    if (!ok) continue;
+    const uint32_t bit = 0x80000000u >> (p & 31);
+    if (inverse)
+      atomicAnd(&bitmap[p >> 5], ~bit);
+    else
+      atomicOr(&bitmap[p >> 5], bit);
+  }
+}
+
+static hipError_t refine_slots(BinDex **bxs, int ncols, int sel, const CODE *predicate,
+                               int64_t s0, int64_t s1, BITS *bitmap, bool inverse) {
+  if (ncols < 1 || ncols > kMaxRefineCols) return hipErrorInvalidValue;
+  RefineCols rc;
+  for (int j = 0; j < ncols; j++) {
+    if (!bxs[j] || !bxs[j]->rawDataInGPU) return hipErrorInvalidValue;
+    rc.raw[j] = bxs[j]->rawDataInGPU;
+    rc.lo[j] = predicate[2 * j];
+    rc.hi[j] = predicate[2 * j + 1];
+  }
+  if (s1 <= s0) return hipSuccess;
+  const int64_t items = s1 - s0;
+  const unsigned grid =
+      (unsigned)std::min<int64_t>((items + kBlock - 1) / kBlock, (int64_t)device_cu_count() * 16);
+  hipLaunchKernelGGL(k_refine_slots, dim3(grid), dim3(kBlock), 0, 0, rc, ncols,
+                     bxs[sel]->dev->sorted_pos, s0, s1, bitmap, inverse);
+  return hipGetLastError();
+}
+
+static inline int64_t block_slot(const Area *a, int block) {
+  return (int64_t)a->start + (int64_t)block * RTS_BLOCK_INIT_SIZE;
+}
+
+static inline int64_t block_end_slot(const Area *a, int block) {
+  return std::min<int64_t>(block_slot(a, block) + RTS_BLOCK_INIT_SIZE, (int64_t)a->start + a->length);
+}
+
+}  // namespace rts
+
+using namespace rts;
+
+extern "C" {
+
+// in_which_area, bindex.cpp:1138-1151 (dup bindex_cuda.cpp:723-731)
+int in_which_area(BinDex *bindex, CODE compare) {
+  if (compare < bindex->areaStartValues[0]) return -1;
+  for (int i = 0; i < kBins; i++) {
+    const CODE sv = bindex->areaStartValues[i];
+    if (sv == compare) return i;
+    if (sv > compare) return i - 1;
+  }
+  return kBins - 1;
+}
+
+// in_which_block, bindex.cpp:1153-1176: the first block starting at `compare`, else the
+// last block starting below it; one block earlier if that block ends with `compare`.
+int in_which_block(Area *area, CODE compare) {
+  int res = area->blockNum - 1;
+  for (int i = 0; i < area->blockNum; i++) {
+    const CODE sv = area->blockFirst[i];
+    if (sv == compare) {
+      res = i;
+      break;
+    }
+    if (sv > compare) {
+      res = i - 1;
+      break;
+    }
+  }
+  if (res > 0 && area->blockLast[res - 1] == compare) res--;
+  return res;
+}
+
+// copy_filter_vector_in_GPU, bindex_cuda.cpp:486-504 (declared bindex.h:145)
+void copy_filter_vector_in_GPU(BinDex *bindex, BITS *dev_bitmap, int k, bool negation) {
+  if (!bindex || !dev_bitmap) return;
+  const uint64_t words = ref_words(bindex->length);
+  hipError_t e;
+  if (k < 0)
+    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0u, 0);
+  else if (k >= kBins - 1)
+    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0xFFFFFFFFu, 0);
+  else
+    e = bitop(negation ? 1 : 0, dev_bitmap, bindex->filterVectorsInGPU[k], nullptr, words, 0u, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  (void)set_error(e, "copy_filter_vector_in_GPU");
+}
+
+// GPURefineAreaWithCuda, cuda_and.cu:510-640.  predicate: host array of 2 * column_num
+// exclusive bounds (lo, hi) per column.  The boundary area of the selected column is found
+// from its upper bound (its lower bound when the upper bound starts an area), then the
+// slots of blocks [0, blockIdx] (inverse: [blockIdx, blockNum)) are refined: rows whose
+// every column lies strictly inside its bounds get their bit set (inverse: cleared).
+// One launch over the contiguous slot range instead of one launch per block.
+hipError_t GPURefineAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CODE *predicate,
+                                 int selected_id, int column_num, bool inverse) {
+  if (!bindexs || !dev_result_bitmap || !predicate || selected_id < 0 ||
+      selected_id >= column_num || !bindexs[selected_id] || !bindexs[selected_id]->dev)
+    return set_error(hipErrorInvalidValue, "GPURefineAreaWithCuda");
+  BinDex *b = bindexs[selected_id];
+  CODE compare = predicate[2 * selected_id + 1];
+  int area = in_which_area(b, compare);
+  if (area >= 0 && b->areaStartValues[area] == compare) {
+    compare = predicate[2 * selected_id];
+    area = in_which_area(b, compare);
+  }
+  if (area < 0) return set_error(hipErrorInvalidValue, "GPURefineAreaWithCuda: bound below data");
+  const Area *a = b->areas[area];
+  const int block = in_which_block(b->areas[area], compare);
+  int64_t s0, s1;
+  if (!inverse) {
+    s0 = block_slot(a, 0);
+    s1 = block_end_slot(a, block);
+  } else {
+    s0 = block_slot(a, block);
+    s1 = block_end_slot(a, a->blockNum - 1);
+  }
+  hipError_t e = refine_slots(bindexs, column_num, selected_id, predicate, s0, s1,
+                              dev_result_bitmap, inverse);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  return set_error(e, "GPURefineAreaWithCuda");
+}
+
+// GPURefineEqAreaWithCuda, cuda_and.cu:642-812: refines from the block of the selected
+// column's smaller bound to the block of its larger bound, across the areas in between.
+hipError_t GPURefineEqAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CODE *predicate,
+                                   int selected_id, int column_num, bool inverse) {
+  if (!bindexs || !dev_result_bitmap || !predicate || selected_id < 0 ||
+      selected_id >= column_num || !bindexs[selected_id] || !bindexs[selected_id]->dev)
+    return set_error(hipErrorInvalidValue, "GPURefineEqAreaWithCuda");
+  BinDex *b = bindexs[selected_id];
+  CODE left = predicate[2 * selected_id], right = predicate[2 * selected_id + 1];
+  if (left > right) std::swap(left, right);
+  const int al = in_which_area(b, left), ar = in_which_area(b, right);
+  if (ar < 0) return set_error(hipErrorInvalidValue, "GPURefineEqAreaWithCuda: bound below data");
+  const int br = in_which_block(b->areas[ar], right);
+  int64_t s0;
+  if (al >= 0) {
+    const int bl = in_which_block(b->areas[al], left);
+    s0 = block_slot(b->areas[al], bl);
+  } else {
+    s0 = block_slot(b->areas[0], 0);
+  }
+  const int64_t s1 = block_end_slot(b->areas[ar], br);
+  hipError_t e = refine_slots(bindexs, column_num, selected_id, predicate, s0, s1,
+                              dev_result_bitmap, inverse);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  return set_error(e, "GPURefineEqAreaWithCuda");
+}
+
+// ------------------------------------------------------------ additive BinDex API
+
+hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, hipStream_t stream) {
+  hipError_t e = bindex_build(bindex, dev_column, n, nullptr, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return set_error(e, "rts_bindex_build");
+}
+
+hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n) {
+  if (!bindex || !host_column || n <= 0) return set_error(hipErrorInvalidValue, "rts_bindex_build_host");
+  CODE *d = nullptr;
+  hipError_t e = hipMalloc((void **)&d, (size_t)n * sizeof(CODE));
+  if (e == hipSuccess) e = hipMemcpy(d, host_column, (size_t)n * sizeof(CODE), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = bindex_build(bindex, d, n, d, 0);
+  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e != hipSuccess && d) (void)hipFree(d);
+  return set_error(e, "rts_bindex_build_host");
+}
+
+void rts_bindex_free(BinDex *bindex) { bindex_free(bindex); }
+
+hipError_t rts_bindex_query(const BinDex *const *bindexes, const CODE *lo, const CODE *hi,
+                            int ncols, BITS *out, unsigned long long *count, hipStream_t stream) {
+  return set_error(bindex_query(bindexes, lo, hi, ncols, out, count, stream), "rts_bindex_query");
+}
+
+hipError_t rts_bindex_export_sorted(const BinDex *bindex, CODE *host_sorted_val,
+                                    int32_t *host_sorted_pos) {
+  if (!bindex || !bindex->dev) return set_error(hipErrorInvalidValue, "rts_bindex_export_sorted");
+  const rts_bindex_dev *d = bindex->dev;
+  hipError_t e = hipSuccess;
+  if (host_sorted_val)
+    e = hipMemcpy(host_sorted_val, d->sorted_val, d->n * sizeof(CODE), hipMemcpyDeviceToHost);
+  if (e == hipSuccess && host_sorted_pos)
+    e = hipMemcpy(host_sorted_pos, d->sorted_pos, d->n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  return set_error(e, "rts_bindex_export_sorted");
+}
+
+long long rts_bindex_refine_entries(const BinDex *bindex) {
+  return bindex && bindex->dev ? (long long)bindex->dev->nb : -1;
+}
+
+long long rts_bindex_device_bytes(const BinDex *bindex) {
+  if (!bindex || !bindex->dev) return -1;
+  const rts_bindex_dev *d = bindex->dev;
+  return (long long)(d->n * 8 + (kBins - 1) * d->fv_stride * 4 + d->nb * 8 +
+                     (kBins + 1) * 4 + kBins * (d->ntiles + 1) * 4);
+}
+
+}  // extern "C"

@@ -31,7 +31,7 @@ def declared_functions():
             src = new
         for stmt in src.split(";"):
             stmt = " ".join(stmt.split())
-            if "(" not in stmt or stmt.startswith("typedef") or stmt.startswith("extern"):
+            if "(" not in stmt or stmt.split(" ")[0] in ("typedef", "extern", "inline"):
                 continue
             m = re.search(r"(\w+)\s*\(", stmt)
             if m:

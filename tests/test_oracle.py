@@ -323,3 +323,56 @@ def test_checks_catch_a_planted_mistake():
     lsb_first = np.packbits(np.pad(x < 25, (0, 0)), bitorder="little").view("<u4")
     assert not np.array_equal(lsb_first, good)
     assert np.array_equal(unpack_rows(good, x.size), x < 25)
+
+
+# ------------------------------------------------------------- refine-area surface
+
+def test_refine_area_closes_the_sieve():
+    """Pins the restatement of GPURefineAreaWithCuda (cuda_and.cu:510-640) by what the
+    reference driver relies on (bindex_cuda.cpp:903-987 with refine_with_Cuda,
+    bindex_cuda.cpp:1506-1660): for `lt c` with c inside area k, the filter vector
+    fv[k-1] plus a non-inverse refine with exclusive bounds (start[k] - 1, c) is exactly
+    x < c; fv[k] plus an inverse refine with (c - 1, start[k+1]) is exactly x < c too."""
+    n = 524288
+    rng = np.random.default_rng(21)
+    x = rng.integers(0, 2 ** 32, n, dtype=np.uint32)
+    ob = O.Bindex(x)
+    sv, fv = ob.start_values, ob.fvs()
+    for _ in range(20):
+        k = int(rng.integers(1, 126))
+        c = int(rng.integers(int(sv[k]) + 1, int(sv[k + 1])))
+        want = pack_rows(x < c)
+        bm = fv[k - 1].copy()
+        assert O.refine_area([ob], [x], [int(sv[k]) - 1, c], 0, False, bm)
+        assert np.array_equal(bm, want), (k, c)
+        bm = fv[k].copy()
+        assert O.refine_area([ob], [x], [c - 1, int(sv[k + 1])], 0, True, bm)
+        assert np.array_equal(bm, want), (k, c)
+    # second column bounds restrict the rows that change
+    y = rng.integers(0, 2 ** 32, n, dtype=np.uint32)
+    oy = O.Bindex(y)
+    k = 50
+    c = int(sv[k]) + (int(sv[k + 1]) - int(sv[k])) // 3
+    bm = np.zeros(ob.words, np.uint32)
+    assert O.refine_area([ob, oy], [x, y], [int(sv[k]) - 1, c, 5, 2 ** 31], 0, False, bm)
+    assert np.array_equal(bm, pack_rows((x >= sv[k]) & (x < c) & (y > 5) & (y < 2 ** 31)))
+
+
+def test_refine_eq_area_covers_the_value_range():
+    """GPURefineEqAreaWithCuda (cuda_and.cu:642-812) refines every block from the smaller
+    bound's block to the larger bound's: on a zero bitmap with exclusive bounds (a, b)
+    spanning several areas it sets exactly a < x < b."""
+    n = 524288
+    rng = np.random.default_rng(22)
+    x = rng.integers(0, 2 ** 32, n, dtype=np.uint32)
+    ob = O.Bindex(x)
+    for a, b in [(int(ob.start_values[3]) + 7, int(ob.start_values[90]) + 99),
+                 (1000, 2 ** 31), (int(ob.start_values[10]) + 1, int(ob.start_values[10]) + 900000)]:
+        bm = np.zeros(ob.words, np.uint32)
+        assert O.refine_area([ob], [x], [a, b], 0, False, bm, eq=True)
+        assert np.array_equal(bm, pack_rows((x > a) & (x < b))), (a, b)
+        # reversed bounds: the block range is taken in order (cuda_and.cu:682-688) but the
+        # row test keeps the given (lo, hi) (cuda_and.cu:83-86), so no row passes
+        bm = np.zeros(ob.words, np.uint32)
+        assert O.refine_area([ob], [x], [b, a], 0, False, bm, eq=True)
+        assert not bm.any()
