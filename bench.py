@@ -93,41 +93,79 @@ def cpu_baseline(x, preds, threads, budget_s=1.5, max_preds=24):
             "counts": counts}
 
 
+class _Slot:
+    """A device pointer into a DeviceBuffer (one count slot per step)."""
+
+    def __init__(self, buf, offset):
+        self.ptr = buf.ptr + offset
+
+
+def timed_loop(R, stream, steps, warmup, launch):
+    """Run launch(i) for warmup + steps iterations on `stream`; HIP events on that stream
+    bracket the timed steps.  Returns milliseconds for the timed steps."""
+    e0, e1 = R.Event(), R.Event()
+    for i in range(warmup):
+        launch(i)
+    stream.synchronize()
+    R.synchronize()
+    e0.record(stream)
+    for i in range(steps):
+        launch(i)
+    e1.record(stream)
+    e1.synchronize()
+    R.synchronize()
+    return e0.elapsed_ms(e1)
+
+
 def bench_scan(args, R):
     n = N_ROWS
     t0 = time.time()
     x = R.generate_column(R.DIST_UNIFORM, DATA_SEED, 0, n)
     dx = R.DeviceBuffer.from_numpy(x)
     preds = config2_predicates()
-    out = R.DeviceBuffer(4 * ((n + 31) // 32))
+    W = (n + 31) // 32
+    out = R.DeviceBuffer(4 * W)
     steps, warmup = args.steps, args.warmup
     counts = R.DeviceBuffer(8 * max(steps, 1))
     stream = R.Stream()
-    e0, e1 = R.Event(), R.Event()
 
-    class Slot:
-        def __init__(self, i):
-            self.ptr = counts.ptr + 8 * i
+    # headline: kernel (a) alone, one predicate per launch, bitmap output
+    def scan(i):
+        lo, hi = preds[i % len(preds)]
+        R.scan_range(dx, n, lo, hi, out, None, stream)
 
-    for i in range(warmup):
+    ms = timed_loop(R, stream, steps, warmup, scan)
+    per_launch_s = ms / 1e3 / steps
+    alg_bytes = n * 4 + W * 4  # column read + bitmap write
+    achieved = alg_bytes / per_launch_s / 1e9
+
+    # same predicates with the popcount (a one-workgroup finalize launch per query)
+    def scan_count(i):
         lo, hi = preds[i % len(preds)]
-        R.scan_range(dx, n, lo, hi, out, Slot(i % max(steps, 1)), stream)
-    stream.synchronize()
-    R.synchronize()
-    e0.record(stream)
-    for i in range(steps):
-        lo, hi = preds[i % len(preds)]
-        R.scan_range(dx, n, lo, hi, out, Slot(i), stream)
-    e1.record(stream)
-    e1.synchronize()
-    R.synchronize()
-    ms = e0.elapsed_ms(e1)
+        R.scan_range(dx, n, lo, hi, out, _Slot(counts, 8 * (i % max(steps, 1))), stream)
+
+    ms_count = timed_loop(R, stream, steps, warmup, scan_count)
     gpu_counts = counts.to_numpy(np.uint64, steps)
 
-    per_launch_s = ms / 1e3 / steps
-    alg_bytes = n * 4 + ((n + 31) // 32) * 4  # column read + bitmap write
-    achieved = alg_bytes / per_launch_s / 1e9
-    traffic = load_traffic("scan_range_1e8_traffic.txt")
+    # BinDex path (kernel b): 128 equal-frequency bins, precomputed filter vectors
+    e0, e1 = R.Event(), R.Event()
+    e0.record(stream)
+    ix = R.BinDexIndex(dx, n, stream)
+    e1.record(stream)
+    e1.synchronize()
+    build_ms = e0.elapsed_ms(e1)
+    bcounts = R.DeviceBuffer(8 * max(steps, 1))
+
+    def bquery(i):
+        lo, hi = preds[i % len(preds)]
+        R.bindex_query([ix], [lo], [hi], out, _Slot(bcounts, 8 * (i % max(steps, 1))), stream)
+
+    ms_b = timed_loop(R, stream, steps, warmup, bquery)
+    b_counts = bcounts.to_numpy(np.uint64, steps)
+    bq_ok = bool(np.array_equal(b_counts, gpu_counts))
+    # algorithmic bytes of one BinDex query: two filter vectors + the two boundary bins'
+    # refine lists (8 B per entry) + the bitmap write
+    b_alg = 3 * W * 4 + 2 * (ix.refine_entries() / 128.0) * 8
 
     cores = min(16, os.cpu_count() or 1)
     cpu = cpu_baseline(x, preds[:steps] if steps else preds, cores)
@@ -149,10 +187,10 @@ def bench_scan(args, R):
         "config": {
             "workload": "plain GPU scan: 1 uint32 column x 1e8 rows, 1000 single-range "
                         "predicates with selectivity U[0,1], one query per launch "
-                        "(rts_scan_range, bitmap + popcount)",
+                        "(rts_scan_range, kernel k_scan_tiles, bitmap output)",
             "rows": n,
             "predicates": N_PRED,
-            "kernel": "k_scan_tiles<1,4,0,true>",
+            "kernel": "k_scan_tiles<1,4,0,false>",
         },
         "roofline": {
             "bound": "hbm",
@@ -160,7 +198,7 @@ def bench_scan(args, R):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": traffic,
+            "traffic": load_traffic("scan_range_1e8_traffic.txt"),
         },
         "cpu_baseline": {
             "value": cpu["rows_per_s"],
@@ -171,9 +209,30 @@ def bench_scan(args, R):
                       "the 1000 predicates over the full 1e8-row column (%.2f s wall)"
                       % (cores, cpu["preds"], cpu["wall_s"]),
         },
-        "parity": {"checked_predicates": len(cpu["counts"]), "popcounts_match": bool(ok)},
+        "extra": {
+            "scan_with_popcount_ms_per_step": ms_count / steps,
+            "bindex_build_ms": build_ms,
+            "bindex_query_ms_per_step": ms_b / steps,
+            "bindex_query_rows_per_s": n * steps / (ms_b / 1e3),
+            "bindex_query_alg_GBps": b_alg / (ms_b / 1e3 / steps) / 1e9,
+            "bindex_device_bytes": ix.device_bytes(),
+        },
+        "parity": {"checked_predicates": len(cpu["counts"]), "popcounts_match_oracle": bool(ok),
+                   "bindex_counts_match_scan": bq_ok},
         "setup_s": round(time.time() - t0, 2),
     }
+
+
+def bench_copycal(args, R):
+    """Calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: five device-to-device
+    copies of exactly 400,000,000 bytes (the HIP runtime's plain copy kernel)."""
+    nbytes = 400_000_000
+    a = R.DeviceBuffer(nbytes).fill(1)
+    b = R.DeviceBuffer(nbytes)
+    for _ in range(5):
+        R.check(R.lib().rts_memcpy_dtod(b.ptr, a.ptr, nbytes), "dtod")
+    R.synchronize()
+    return {"workload": "copycal", "copies": 5, "bytes_per_copy": nbytes}
 
 
 def main():
@@ -181,7 +240,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="scan", choices=["scan"])
+    ap.add_argument("--workload", default="scan", choices=["scan", "copycal"])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
@@ -193,7 +252,7 @@ def main():
     if R.lib().rts_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    res = bench_scan(args, R)
+    res = bench_copycal(args, R) if args.workload == "copycal" else bench_scan(args, R)
     print(json.dumps(res), flush=True)
 
 

@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
     }
     __syncthreads();
   }
-  if (COUNT) count_finish(pc, sink);
+  if (COUNT) count_block_partial(pc, sink);
 }
 
 static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
@@ -141,10 +141,13 @@ static hipError_t launch_query(const QueryArgs<C> &q, const rts_bindex_dev *d, B
   const bool vec = ((uintptr_t)out & 15) == 0;
   const unsigned grid = (unsigned)std::min<int64_t>(d->ntiles, (int64_t)device_cu_count() * 32);
   if (count) {
-    CountSink sink{count, stream_count_scratch(s)};
-    if (!sink.scr) return hipErrorOutOfMemory;
+    CountSink sink{count, stream_count_partials(s)};
+    if (!sink.partials) return hipErrorOutOfMemory;
     hipLaunchKernelGGL((k_bindex_query<C, MODE, true>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
                        d->words, d->ntiles, out, vec, sink);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    return launch_count_finalize(sink, grid, s);
   } else {
     hipLaunchKernelGGL((k_bindex_query<C, MODE, false>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
                        d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr});

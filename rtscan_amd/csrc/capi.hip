@@ -54,9 +54,9 @@ int device_cu_count() {
   return cache[dev];
 }
 
-CountScratch *stream_count_scratch(hipStream_t stream) {
+uint32_t *stream_count_partials(hipStream_t stream) {
   static std::mutex mu;
-  static std::map<std::tuple<int, uintptr_t, std::thread::id>, CountScratch *> table;
+  static std::map<std::tuple<int, uintptr_t, std::thread::id>, uint32_t *> table;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
   // hipStreamPerThread names a different stream on every host thread
@@ -66,11 +66,46 @@ CountScratch *stream_count_scratch(hipStream_t stream) {
   std::lock_guard<std::mutex> lock(mu);
   auto it = table.find(key);
   if (it != table.end()) return it->second;
-  CountScratch *p = nullptr;
-  if (hipMalloc((void **)&p, 256) != hipSuccess) return nullptr;
-  if (hipMemset(p, 0, 256) != hipSuccess || hipStreamSynchronize(0) != hipSuccess) return nullptr;
+  uint32_t *p = nullptr;
+  if (hipMalloc((void **)&p, kMaxCountBlocks * sizeof(uint32_t)) != hipSuccess) return nullptr;
   table[key] = p;
   return p;
+}
+
+// One workgroup of 1024 threads; each thread keeps 8 independent loads in flight so the
+// sum over tens of thousands of partials costs a few memory latencies, not one per slot.
+constexpr int kFinalizeThreads = 1024;
+
+__global__ __launch_bounds__(kFinalizeThreads) void k_count_finalize(const uint32_t *partials,
+                                                                     unsigned nblocks,
+                                                                     unsigned long long *out) {
+  __shared__ unsigned long long part[kFinalizeThreads / kWave];
+  unsigned long long s = 0;
+  unsigned i = threadIdx.x;
+  for (; i + 7 * kFinalizeThreads < nblocks; i += 8 * kFinalizeThreads) {
+    uint32_t v[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) v[k] = partials[i + k * kFinalizeThreads];
+#pragma unroll
+    for (int k = 0; k < 8; k++) s += v[k];
+  }
+  for (; i < nblocks; i += kFinalizeThreads) s += partials[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int k = 0; k < kFinalizeThreads / kWave; k++) t += part[k];
+    *out = t;
+  }
+}
+
+hipError_t launch_count_finalize(const CountSink &sink, unsigned nblocks, hipStream_t stream) {
+  if (!sink.out) return hipSuccess;
+  hipLaunchKernelGGL(k_count_finalize, dim3(1), dim3(kFinalizeThreads), 0, stream, sink.partials,
+                     nblocks, sink.out);
+  return hipGetLastError();
 }
 
 // Reference wrappers: legacy null stream, synchronous (cuda_and.cu ends every wrapper with

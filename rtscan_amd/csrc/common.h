@@ -70,53 +70,41 @@ __device__ __forceinline__ uint32_t store_word(BITS *out, uint64_t w, uint32_t v
   return v;
 }
 
-// Per-stream device scratch for popcount results.  A kernel's blocks add their partial
-// counts into `acc` and take a ticket; the last block moves the total to the caller's
-// output and resets both fields, so the scratch is zero again when the kernel ends and
-// the output needs no clearing launch.  Launches on one stream are ordered, so one
-// scratch per stream suffices.
-struct CountScratch {
-  unsigned long long acc;
-  unsigned int ticket;
-  unsigned int pad;
-};
+// Popcounts: every block of a counting kernel stores its partial count in its own slot of
+// a per-stream scratch array (plain stores, no same-address atomics: thousands of blocks
+// serialising on one counter cost more than the scan itself), then a one-block kernel on
+// the same stream sums the slots into the caller's output.
+constexpr int kMaxCountBlocks = 1 << 16;
 
 struct CountSink {
   unsigned long long *out;  // nullptr: no count wanted
-  CountScratch *scr;
+  uint32_t *partials;       // kMaxCountBlocks slots
 };
 
-// Adds this block's threads' counts `pc` into the sink; the last block of the grid writes
-// the total.  Every block of the grid must call it exactly once.
-__device__ __forceinline__ void count_finish(uint32_t pc, CountSink sink) {
+// Block-wide sum of the threads' counts into partials[blockIdx.x].  Every block of the
+// grid must call it exactly once; the grid must not exceed kMaxCountBlocks blocks.
+__device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink) {
   __shared__ uint32_t part[kBlock / kWave];
-  __shared__ bool last;
   pc = wave_sum(pc);
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   if (lane == 0) part[wv] = pc;
   __syncthreads();
   if (threadIdx.x == 0) {
-    unsigned long long s = 0;
+    uint32_t s = 0;
     for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += part[i];
-    if (s) atomicAdd(&sink.scr->acc, s);
-    __threadfence();
-    const unsigned int t = atomicAdd(&sink.scr->ticket, 1u);
-    last = t == gridDim.x - 1;
-  }
-  __syncthreads();
-  if (last && threadIdx.x == 0) {
-    __threadfence();
-    const unsigned long long total = atomicExch(&sink.scr->acc, 0ull);
-    atomicExch(&sink.scr->ticket, 0u);
-    *sink.out = total;
+    sink.partials[blockIdx.x] = s;
   }
 }
+
+// Scratch of (current device, stream) for count partials, allocated on first use.
+uint32_t *stream_count_partials(hipStream_t stream);
+
+// Sum partials[0, nblocks) into *out on `stream` (one small launch).
+hipError_t launch_count_finalize(const CountSink &sink, unsigned nblocks, hipStream_t stream);
 
 // Number of compute units of the current device (cached per device).
 int device_cu_count();
 
-// The count scratch of (current device, stream), allocated and zeroed on first use.
-CountScratch *stream_count_scratch(hipStream_t stream);
 
 // Record `e` as the calling thread's last error; returns e.
 hipError_t set_error(hipError_t e, const char *what);

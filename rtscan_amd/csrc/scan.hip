@@ -84,7 +84,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_tiles(ColPreds<C> p, uint64_t n
     for (uint64_t r0 = begin + 64 * (threadIdx.x / kWave); r0 < n; r0 += kBlock)
       pc += scan_chunk64<C, MODE>(p, r0, n, out, lane);
   }
-  if (COUNT) count_finish(pc, sink);
+  if (COUNT) count_block_partial(pc, sink);
 }
 
 // Any alignment: one row per lane over all n rows.
@@ -98,7 +98,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_rows(ColPreds<C> p, uint64_t n,
   uint32_t pc = 0;
   for (uint64_t ch = wave; ch < nchunks; ch += nwaves)
     pc += scan_chunk64<C, MODE>(p, ch * 64, n, out, lane);
-  if (COUNT) count_finish(pc, sink);
+  if (COUNT) count_block_partial(pc, sink);
 }
 
 __global__ void k_zero_u64(unsigned long long *p) { *p = 0ull; }
@@ -207,7 +207,7 @@ __global__ __launch_bounds__(kBlock) void k_and_popcount(BitmapSet s, int nb, ui
     if (out) out[i] = acc;
     if (COUNT) pc += __popc(acc);
   }
-  if (COUNT) count_finish(pc, sink);
+  if (COUNT) count_block_partial(pc, sink);
 }
 
 // ------------------------------------------------------------------ host launchers
@@ -222,6 +222,11 @@ static unsigned grid_for(uint64_t work_items, unsigned per_block, unsigned cap) 
 }
 
 static unsigned stream_grid_cap() { return (unsigned)std::max(1, device_cu_count()) * 8u; }
+
+// The scan kernel runs one 1024-row tile per wave without a grid-stride loop up to this
+// many workgroups (16 waves per CU of 256 CUs in flight with room to spare), which measured
+// faster than a persistent grid of 8 workgroups per CU.
+static unsigned scan_grid_cap() { return (unsigned)std::min<int>(kMaxCountBlocks, std::max(1, device_cu_count()) * 128); }
 
 template <int OP>
 static hipError_t launch_bitop(BITS *dst, const BITS *a, const BITS *b, uint64_t words,
@@ -243,25 +248,28 @@ static hipError_t launch_scan_c(const ColPreds<C> &p, uint64_t n, BITS *out, Cou
                                 hipStream_t s) {
   bool vec = true;
   for (int c = 0; c < C; c++) vec = vec && aligned16(p.col[c]);
+  unsigned grid;
   if (vec) {
     const uint64_t ntiles = n / (256ull * kTileU);
-    const unsigned grid = grid_for(std::max<uint64_t>(ntiles, 1), kBlock / kWave, stream_grid_cap());
+    grid = grid_for(std::max<uint64_t>(ntiles, 1), kBlock / kWave, scan_grid_cap());
     hipLaunchKernelGGL((k_scan_tiles<C, kTileU, MODE, COUNT>), dim3(grid), dim3(kBlock), 0, s, p,
                        n, out, sink);
   } else {
-    const unsigned grid = grid_for((n + 63) / 64, kBlock / kWave, stream_grid_cap());
+    grid = grid_for((n + 63) / 64, kBlock / kWave, stream_grid_cap());
     hipLaunchKernelGGL((k_scan_rows<C, MODE, COUNT>), dim3(grid), dim3(kBlock), 0, s, p, n, out,
                        sink);
   }
-  return hipGetLastError();
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || !COUNT) return e;
+  return launch_count_finalize(sink, grid, s);
 }
 
 template <int C, int MODE>
 static hipError_t launch_scan(const ColPreds<C> &p, uint64_t n, BITS *out,
                               unsigned long long *count, hipStream_t s) {
   if (count) {
-    CountSink sink{count, stream_count_scratch(s)};
-    if (!sink.scr) return hipErrorOutOfMemory;
+    CountSink sink{count, stream_count_partials(s)};
+    if (!sink.partials) return hipErrorOutOfMemory;
     return launch_scan_c<C, MODE, true>(p, n, out, sink, s);
   }
   return launch_scan_c<C, MODE, false>(p, n, out, CountSink{nullptr, nullptr}, s);
@@ -334,8 +342,8 @@ hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS 
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, count);
     return hipGetLastError();
   }
-  CountScratch *scr = count ? stream_count_scratch(s) : nullptr;
-  if (count && !scr) return hipErrorOutOfMemory;
+  uint32_t *partials = count ? stream_count_partials(s) : nullptr;
+  if (count && !partials) return hipErrorOutOfMemory;
   // Chains of more than kMaxAnd inputs: AND the first kMaxAnd into `out`, continue from it.
   int done = 0;
   while (true) {
@@ -351,7 +359,7 @@ hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS 
     for (int i = 0; i < k; i++) vec = vec && aligned16(set.bm[i]);
     const unsigned grid = grid_for(vec ? (words + 3) / 4 : words, kBlock, stream_grid_cap());
     const uint32_t lm = tail_mask(n);
-    const CountSink sink{last ? count : nullptr, last ? scr : nullptr};
+    const CountSink sink{last ? count : nullptr, last ? partials : nullptr};
 #define RTS_LAUNCH_AND(V, C)                                                                    \
   hipLaunchKernelGGL((k_and_popcount<V, C>), dim3(grid), dim3(kBlock), 0, s, set, k, words, lm, \
                      out, sink)
@@ -362,6 +370,7 @@ hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS 
     }
 #undef RTS_LAUNCH_AND
     hipError_t e = hipGetLastError();
+    if (e == hipSuccess && sink.out) e = launch_count_finalize(sink, grid, s);
     if (e != hipSuccess || last) return e;
   }
 }
