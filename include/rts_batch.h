@@ -1,0 +1,92 @@
+/*
+ * rts_batch.h -- batched evaluation of many range queries in one persistent launch
+ * (additive: the reference runs one query at a time, bindex_cuda.cpp:1827-1995).
+ *
+ * A query is an AND of closed ranges over 1..RTS_BATCH_MAX_COLS columns of a table whose
+ * columns all have n rows.  Each query-column is evaluated from the column's BinDex (filter
+ * vectors + boundary-bin refine lists) or directly from its values; RTS_EVAL_AUTO takes the
+ * BinDex when the column has one.  Query q's bitmap is written at out + q * out_stride_words
+ * (MSB-first words, bits past row n cleared) and its popcount at counts[q] (device).
+ *
+ * The launch is one persistent kernel over (row tile, query group) work items, assigned so
+ * that one tile's work items run on one XCD back to back, plus one small kernel summing the
+ * per-(query, tile) popcounts.  rts_batch_plan_graph_launch captures both in a hipGraph and
+ * replays it; queries set with rts_batch_plan_set_queries live in device memory at a fixed
+ * address, so a replay evaluates the current set.
+ */
+#ifndef RTS_BATCH_H_
+#define RTS_BATCH_H_
+
+#include "rts_bindex.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RTS_BATCH_MAX_COLS 4
+
+enum rts_eval_mode { RTS_EVAL_AUTO = 0, RTS_EVAL_BINDEX = 1, RTS_EVAL_DIRECT = 2 };
+
+typedef struct rts_query_spec {
+  int ncols;                      /* 1..RTS_BATCH_MAX_COLS */
+  int column[RTS_BATCH_MAX_COLS]; /* table column of each range */
+  CODE lo[RTS_BATCH_MAX_COLS];    /* closed range lo <= x <= hi; lo > hi is empty */
+  CODE hi[RTS_BATCH_MAX_COLS];
+  int mode[RTS_BATCH_MAX_COLS];   /* enum rts_eval_mode */
+} rts_query_spec;
+
+/* Launch shape: tile_words output words per work item (1024 or 2048, i.e. 32768 or 65536
+ * rows), queries_per_group queries per work item (1..64), blocks_per_cu persistent
+ * workgroups per compute unit (1..16). */
+typedef struct rts_batch_config {
+  int tile_words;
+  int queries_per_group;
+  int blocks_per_cu;
+} rts_batch_config;
+
+typedef struct rts_batch_plan rts_batch_plan;
+
+void rts_batch_default_config(rts_batch_config *cfg);
+
+/* columns: host array of ncols device column pointers; indexes: host array of ncols BinDex
+ * pointers (entries or the whole array may be NULL: those columns are evaluated directly).
+ * Allocates the device query table (max_queries) and per-(query, tile) count partials. */
+hipError_t rts_batch_plan_create(rts_batch_plan **plan, const CODE *const *columns,
+                                 const BinDex *const *indexes, int ncols, int64_t n,
+                                 int max_queries, const rts_batch_config *cfg, hipStream_t stream);
+/* Change the launch shape (a new tile size re-tiles the list directories and drops the
+ * current queries: set them again). */
+hipError_t rts_batch_plan_set_config(rts_batch_plan *plan, const rts_batch_config *cfg,
+                                     hipStream_t stream);
+/* Plan nq <= max_queries queries (host array) and upload them; returns after the upload. */
+hipError_t rts_batch_plan_set_queries(rts_batch_plan *plan, const rts_query_spec *queries,
+                                      int nq, hipStream_t stream);
+/* Evaluate the current queries: out holds nq bitmaps of out_stride_words words each;
+ * counts (device, nq entries) may be NULL.  Asynchronous on stream. */
+hipError_t rts_batch_plan_launch(rts_batch_plan *plan, BITS *out, int64_t out_stride_words,
+                                 unsigned long long *counts, hipStream_t stream);
+/* As rts_batch_plan_launch through a captured hipGraph (captured on first use and again
+ * when out / counts / stride / query count change).  stream must not be the null stream. */
+hipError_t rts_batch_plan_graph_launch(rts_batch_plan *plan, BITS *out, int64_t out_stride_words,
+                                       unsigned long long *counts, hipStream_t stream);
+/* Planner statistics of the current queries: query-columns on the BinDex path, on the direct
+ * path, BinDex columns whose range lies inside one bin (no filter vector read); row tiles. */
+void rts_batch_plan_stats(const rts_batch_plan *plan, int *bindex_cols, int *direct_cols,
+                          int *narrow_cols, long long *ntiles);
+/* Rows, table columns and current query count of a plan. */
+void rts_batch_plan_info(const rts_batch_plan *plan, long long *n, int *ncols, int *nq);
+void rts_batch_plan_free(rts_batch_plan *plan);
+
+/* Autotuner: time every candidate launch shape on the plan's current queries (median of
+ * `reps` launches each) and keep the fastest in the plan.  With cache_path non-NULL the
+ * result is read from / appended to that file, keyed by device, n, column count and query
+ * count, so later runs skip the search.  best_ms (may be NULL) receives the winner's time. */
+hipError_t rts_batch_autotune(rts_batch_plan *plan, BITS *out, int64_t out_stride_words,
+                              const rts_batch_config *candidates, int ncandidates, int reps,
+                              const char *cache_path, rts_batch_config *chosen, float *best_ms,
+                              hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RTS_BATCH_H_ */

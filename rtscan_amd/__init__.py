@@ -90,6 +90,21 @@ SIGNATURES = {
     "rts_query_file_free": (None, [_vp]),
     "rts_data_file_read": (_int, [ctypes.c_char_p, _int, ctypes.c_int64, _vp]),
     "rts_data_file_write": (_int, [ctypes.c_char_p, _int, ctypes.c_int64, _vp]),
+    # batched queries
+    "rts_batch_default_config": (None, [_vp]),
+    "rts_batch_plan_create": (_err, [ctypes.POINTER(_vp), _vp, _vp, _int, ctypes.c_int64, _int,
+                                     _vp, _vp]),
+    "rts_batch_plan_set_config": (_err, [_vp, _vp, _vp]),
+    "rts_batch_plan_set_queries": (_err, [_vp, _vp, _int, _vp]),
+    "rts_batch_plan_launch": (_err, [_vp, _vp, ctypes.c_int64, _vp, _vp]),
+    "rts_batch_plan_graph_launch": (_err, [_vp, _vp, ctypes.c_int64, _vp, _vp]),
+    "rts_batch_plan_stats": (None, [_vp, ctypes.POINTER(_int), ctypes.POINTER(_int),
+                                    ctypes.POINTER(_int), ctypes.POINTER(ctypes.c_longlong)]),
+    "rts_batch_plan_info": (None, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_int),
+                                   ctypes.POINTER(_int)]),
+    "rts_batch_plan_free": (None, [_vp]),
+    "rts_batch_autotune": (_err, [_vp, _vp, ctypes.c_int64, _vp, _int, _int, ctypes.c_char_p, _vp,
+                                  ctypes.POINTER(ctypes.c_float), _vp]),
     # generators
     "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
                                    _vp, _int]),
@@ -406,3 +421,112 @@ def read_data_file(path, ncols, n):
     if rc:
         raise OSError("rts_data_file_read failed: %d" % rc)
     return cols
+
+
+BATCH_MAX_COLS = 4
+EVAL_AUTO, EVAL_BINDEX, EVAL_DIRECT = 0, 1, 2
+
+
+class QuerySpec(ctypes.Structure):
+    _fields_ = [("ncols", ctypes.c_int), ("column", ctypes.c_int * BATCH_MAX_COLS),
+                ("lo", ctypes.c_uint32 * BATCH_MAX_COLS), ("hi", ctypes.c_uint32 * BATCH_MAX_COLS),
+                ("mode", ctypes.c_int * BATCH_MAX_COLS)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("tile_words", ctypes.c_int), ("queries_per_group", ctypes.c_int),
+                ("blocks_per_cu", ctypes.c_int)]
+
+    def astuple(self):
+        return (self.tile_words, self.queries_per_group, self.blocks_per_cu)
+
+
+def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None):
+    c = BatchConfig()
+    lib().rts_batch_default_config(ctypes.byref(c))
+    if tile_words is not None:
+        c.tile_words = tile_words
+    if queries_per_group is not None:
+        c.queries_per_group = queries_per_group
+    if blocks_per_cu is not None:
+        c.blocks_per_cu = blocks_per_cu
+    return c
+
+
+def query_specs(queries):
+    """queries: list of lists of (column, lo, hi[, mode]) -> ctypes array of QuerySpec."""
+    arr = (QuerySpec * max(1, len(queries)))()
+    for i, q in enumerate(queries):
+        if not 1 <= len(q) <= BATCH_MAX_COLS:
+            raise ValueError("a query has 1..%d columns" % BATCH_MAX_COLS)
+        arr[i].ncols = len(q)
+        for c, item in enumerate(q):
+            col, lo, hi = item[:3]
+            arr[i].column[c] = col
+            arr[i].lo[c] = lo
+            arr[i].hi[c] = hi
+            arr[i].mode[c] = item[3] if len(item) > 3 else EVAL_AUTO
+    return arr
+
+
+class BatchPlan:
+    """Batched evaluation plan over a table (include/rts_batch.h)."""
+
+    def __init__(self, columns, indexes, n, max_queries, config=None, stream=None):
+        self.columns, self.indexes = columns, indexes  # keep device objects alive
+        cols = (_vp * len(columns))(*[c.ptr for c in columns])
+        idx = (_vp * len(columns))(*[(ix.ptr if ix is not None else None) for ix in indexes])
+        h = _vp()
+        cfg = ctypes.byref(config) if config is not None else None
+        check(lib().rts_batch_plan_create(ctypes.byref(h), cols, idx, len(columns), n, max_queries,
+                                          cfg, _sh(stream)), "rts_batch_plan_create")
+        self.h = h.value
+        self.n = n
+
+    def set_config(self, config, stream=None):
+        check(lib().rts_batch_plan_set_config(self.h, ctypes.byref(config), _sh(stream)),
+              "rts_batch_plan_set_config")
+
+    def set_queries(self, queries, stream=None):
+        arr = query_specs(queries)
+        check(lib().rts_batch_plan_set_queries(self.h, arr, len(queries), _sh(stream)),
+              "rts_batch_plan_set_queries")
+
+    def launch(self, out, stride_words, counts=None, stream=None):
+        check(lib().rts_batch_plan_launch(self.h, out.ptr, stride_words,
+                                          counts.ptr if counts else None, _sh(stream)),
+              "rts_batch_plan_launch")
+
+    def graph_launch(self, out, stride_words, counts=None, stream=None):
+        check(lib().rts_batch_plan_graph_launch(self.h, out.ptr, stride_words,
+                                                counts.ptr if counts else None, _sh(stream)),
+              "rts_batch_plan_graph_launch")
+
+    def autotune(self, out, stride_words, candidates, reps=3, cache_path=None, stream=None):
+        arr = (BatchConfig * len(candidates))(*candidates)
+        chosen = BatchConfig()
+        ms = ctypes.c_float()
+        check(lib().rts_batch_autotune(self.h, out.ptr, stride_words, arr, len(candidates), reps,
+                                       cache_path.encode() if cache_path else None,
+                                       ctypes.byref(chosen), ctypes.byref(ms), _sh(stream)),
+              "rts_batch_autotune")
+        return chosen, ms.value
+
+    def stats(self):
+        b, d, nr = _int(), _int(), _int()
+        nt = ctypes.c_longlong()
+        lib().rts_batch_plan_stats(self.h, ctypes.byref(b), ctypes.byref(d), ctypes.byref(nr),
+                                   ctypes.byref(nt))
+        return {"bindex_cols": b.value, "direct_cols": d.value, "narrow_cols": nr.value,
+                "tiles": nt.value}
+
+    def free(self):
+        if getattr(self, "h", None):
+            lib().rts_batch_plan_free(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass

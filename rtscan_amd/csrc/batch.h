@@ -1,0 +1,13 @@
+// batch.h -- constants shared by the batched-query kernels and their host planner.
+#pragma once
+
+#include "common.h"
+#include "rts_batch.h"
+
+namespace rts {
+
+constexpr int kBatchMaxCols = RTS_BATCH_MAX_COLS;  // columns per query
+constexpr int kBatchMaxGroup = 64;                 // queries per work item, upper bound
+constexpr int kXcds = 8;                           // MI355X: 8 XCDs, workgroup b -> XCD b % 8
+
+}  // namespace rts
