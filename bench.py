@@ -223,6 +223,214 @@ def bench_scan(args, R):
     }
 
 
+TABLE_SEED = 3
+TABLE_KINDS = [  # (distribution, p0, p1): uniform, normal(2^31, 2^28), Zipf 1.1 and 1.5
+    (0, 0.0, 0.0), (1, 2.0 ** 31, 2.0 ** 28), (2, 1.1, 2.0 ** 20), (2, 1.5, 2.0 ** 20)]
+
+
+class Table:
+    """The 1e8-row x 4-column table of the conjunction and batched workloads: uniform,
+    normal (mean 2^31, sd 2^28, clipped), Zipf s=1.1 and s=1.5 over 2^20 ranks mapped to
+    random uint32 keys; resident in HBM with a BinDex per column."""
+
+    def __init__(self, R, n=N_ROWS, seed=TABLE_SEED, stream=None):
+        self.n = n
+        self.cols = [R.generate_column(k, seed, c, n, p0, p1)
+                     for c, (k, p0, p1) in enumerate(TABLE_KINDS)]
+        self.dcols = [R.DeviceBuffer.from_numpy(c) for c in self.cols]
+        e0, e1 = R.Event(), R.Event()
+        st = stream or R.Stream()
+        e0.record(st)
+        self.idx = [R.BinDexIndex(d, n, st) for d in self.dcols]
+        e1.record(st)
+        e1.synchronize()
+        self.build_ms = e0.elapsed_ms(e1)
+        rng = np.random.default_rng(seed)
+        pick = rng.integers(0, n, 1_000_000)
+        self.samples = [np.sort(c[pick]) for c in self.cols]  # for quantile endpoints
+
+    def range_for(self, c, sel, rng):
+        """Closed range on column c whose selectivity is about `sel`: endpoints at sample
+        quantiles q0 and q0 + sel."""
+        s = self.samples[c]
+        q0 = float(rng.random()) * (1.0 - sel)
+        lo = int(s[int(q0 * (s.size - 1))])
+        hi = int(s[min(s.size - 1, int((q0 + sel) * (s.size - 1)))])
+        return lo, hi
+
+
+def bench_conj(R, table, nq=30, seed=33):
+    """Config-3 conjunctions over 2, 3 and 4 columns, per-column selectivity in {0.01, 0.1,
+    0.5}: the fused direct kernel (k_scan_tiles<C>), the fused BinDex kernel
+    (k_bindex_query<C>), and the reference's unfused pattern (one scan per column, then a
+    GPUbitAndWithCuda chain as merge_with_GPU does, bindex_cuda.cpp:1693-1702)."""
+    n = table.n
+    W = (n + 31) // 32
+    rng = np.random.default_rng(seed)
+    stream = R.Stream()
+    out = R.DeviceBuffer(4 * W)
+    tmp = [R.DeviceBuffer(4 * W) for _ in range(4)]
+    cnt = R.DeviceBuffer(8)
+    res = {}
+    for k in (2, 3, 4):
+        qs = []
+        for _ in range(nq):
+            cs = list(rng.choice(4, k, replace=False))
+            rngs = [table.range_for(c, [0.01, 0.1, 0.5][int(rng.integers(3))], rng) for c in cs]
+            qs.append((cs, rngs))
+        counts = {}
+
+        def run(kind, q):
+            cs, rr = q
+            lo = [a for a, _ in rr]
+            hi = [b for _, b in rr]
+            if kind == "direct":
+                R.conjunction_scan([table.dcols[c] for c in cs], lo, hi, n, out, cnt, stream)
+            elif kind == "bindex":
+                R.bindex_query([table.idx[c] for c in cs], lo, hi, out, cnt, stream)
+            else:
+                for j, c in enumerate(cs):
+                    R.scan_range(table.dcols[c], n, lo[j], hi[j], tmp[j], None, stream)
+                stream.synchronize()
+                for j in range(1, len(cs)):
+                    R.check(R.lib().GPUbitAndWithCuda(tmp[0].ptr, tmp[j].ptr, n), "and")
+                R.bitmap_popcount(tmp[0], n, cnt, stream)
+
+        entry = {}
+        for kind in ("direct", "bindex", "unfused"):
+            for q in qs[:2]:
+                run(kind, q)
+            stream.synchronize()
+            R.synchronize()
+            got = []
+            t0 = time.perf_counter()
+            for q in qs:
+                run(kind, q)
+                stream.synchronize()
+                got.append(int(cnt.to_numpy(np.uint64, 1)[0]))
+            dt = (time.perf_counter() - t0) / nq
+            counts[kind] = got
+            entry[kind + "_us_per_query"] = dt * 1e6
+        entry["direct_alg_GBps"] = (k * 4 * n + 4 * W) / (entry["direct_us_per_query"] * 1e-6) / 1e9
+        entry["counts_agree"] = counts["direct"] == counts["bindex"] == counts["unfused"]
+        entry["realised_selectivity_median"] = float(np.median(counts["direct"])) / n
+        res["cols%d" % k] = entry
+    # one query per column count against the oracle's conjunction on the host
+    import oracle
+
+    ok = True
+    for k in (2, 4):
+        cs = list(range(k))
+        rr = [table.range_for(c, 0.5, rng) for c in cs]
+        R.conjunction_scan([table.dcols[c] for c in cs], [a for a, _ in rr], [b for _, b in rr], n,
+                           out, cnt, stream)
+        stream.synchronize()
+        preds = [(oracle.BT, [a - 1], [b + 1]) if a > 0 and b < 0xFFFFFFFF else
+                 (oracle.GE, [a], None) for (a, b) in rr]
+        truth = oracle.conjunction([table.cols[c] for c in cs], preds, threads=16)
+        ok = ok and bool(np.array_equal(out.to_numpy(np.uint32, W), truth))
+    res["oracle_bitmaps_match"] = ok
+    return res
+
+
+def batch_queries(table, nq, seed, direct_share):
+    """Config-4 queries: 1..4 columns chosen uniformly, per-column selectivity log-uniform in
+    [1e-5, 0.9]; a share of the query-columns is evaluated directly (mode 2)."""
+    rng = np.random.default_rng(seed)
+    qs = []
+    for _ in range(nq):
+        k = int(rng.integers(1, 5))
+        cs = rng.choice(4, k, replace=False)
+        q = []
+        for c in cs:
+            sel = float(np.exp(rng.uniform(np.log(1e-5), np.log(0.9))))
+            lo, hi = table.range_for(int(c), sel, rng)
+            q.append((int(c), lo, hi, 2 if rng.random() < direct_share else 0))
+        qs.append(q)
+    return qs
+
+
+def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cache=None):
+    """Config 4: nq mixed queries in one persistent hipGraph-captured launch, launch shape
+    from the autotuner, against the plain one-query-per-launch paths (direct conjunction,
+    and BinDex) on the same GPU."""
+    n = table.n
+    W = (n + 31) // 32
+    stride = (W + 3) // 4 * 4
+    qs = batch_queries(table, nq, seed, direct_share)
+    stream = R.Stream()
+    out = R.DeviceBuffer(4 * stride * nq)
+    counts = R.DeviceBuffer(8 * nq)
+    plan = R.BatchPlan(table.dcols, table.idx, n, nq, None, stream)
+    plan.set_queries(qs, stream)
+    cands = [R.batch_config(t, g, b) for t in (1024, 2048) for g in (8, 16, 32) for b in (2, 4)]
+    t0 = time.perf_counter()
+    chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,
+                                    stream=stream)
+    tune_s = time.perf_counter() - t0
+    plan.graph_launch(out, stride, counts, stream)  # capture + first run
+    stream.synchronize()
+    ms = timed_loop(R, stream, steps, 1,
+                    lambda i: plan.graph_launch(out, stride, counts, stream))
+    batch_counts = counts.to_numpy(np.uint64, nq)
+
+    # plain one-query-per-launch paths over the same queries
+    one = R.DeviceBuffer(4 * W)
+    c1 = R.DeviceBuffer(8 * nq)
+
+    class Slot:
+        def __init__(self, i):
+            self.ptr = c1.ptr + 8 * i
+
+    def plain_direct(i):
+        q = qs[i]
+        R.conjunction_scan([table.dcols[c] for c, _, _, _ in q], [lo for _, lo, _, _ in q],
+                           [hi for _, _, hi, _ in q], n, one, Slot(i), stream)
+
+    ms_plain = timed_loop(R, stream, nq, 8, plain_direct)
+    plain_counts = c1.to_numpy(np.uint64, nq)
+
+    def plain_bindex(i):
+        q = qs[i]
+        R.bindex_query([table.idx[c] for c, _, _, _ in q], [lo for _, lo, _, _ in q],
+                       [hi for _, _, hi, _ in q], one, Slot(i), stream)
+
+    ms_plain_b = timed_loop(R, stream, nq, 8, plain_bindex)
+    plain_b_counts = c1.to_numpy(np.uint64, nq)
+    # full bitmaps of a sample of queries: batched == one-query-per-launch
+    same = True
+    for i in range(0, nq, max(1, nq // 8)):
+        plain_direct(i)
+        stream.synchronize()
+        a = one.to_numpy(np.uint32, W)
+        b = np.empty(W, np.uint32)
+        R.check(R.lib().rts_memcpy_dtoh(b.ctypes.data, out.ptr + 4 * stride * i, 4 * W), "dtoh")
+        same = same and bool(np.array_equal(a, b))
+    st = plan.stats()
+    qps = nq * steps / (ms / 1e3)
+    qps_plain = nq / (ms_plain / 1e3)
+    qps_plain_b = nq / (ms_plain_b / 1e3)
+    out_bytes = nq * W * 4
+    plan.free()
+    return {
+        "queries": nq,
+        "batch_ms": ms / steps,
+        "batch_queries_per_s": qps,
+        "plain_direct_queries_per_s": qps_plain,
+        "plain_bindex_queries_per_s": qps_plain_b,
+        "speedup_vs_plain_direct": qps / qps_plain,
+        "speedup_vs_plain_bindex": qps / qps_plain_b,
+        "bitmap_write_GBps": out_bytes / (ms / steps / 1e3) / 1e9,
+        "autotune": {"chosen": list(chosen.astuple()), "best_ms": tune_ms, "search_s": tune_s,
+                     "candidates": len(cands)},
+        "planner": st,
+        "direct_share": direct_share,
+        "counts_match_plain": bool(np.array_equal(batch_counts, plain_counts)) and
+                              bool(np.array_equal(batch_counts, plain_b_counts)),
+        "sampled_bitmaps_match_plain": same,
+    }
+
+
 def bench_copycal(args, R):
     """Calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: five device-to-device
     copies of exactly 400,000,000 bytes (the HIP runtime's plain copy kernel)."""
@@ -240,7 +448,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="scan", choices=["scan", "copycal"])
+    ap.add_argument("--workload", default="scan", choices=["scan", "copycal", "conj", "batch"])
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the conjunction and batched legs of the default run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
@@ -252,7 +462,27 @@ def main():
     if R.lib().rts_device_count() < 1:
         print("bench.py: no GPU visible", file=sys.stderr)
         sys.exit(1)
-    res = bench_copycal(args, R) if args.workload == "copycal" else bench_scan(args, R)
+    if args.workload == "copycal":
+        res = bench_copycal(args, R)
+    elif args.workload in ("conj", "batch"):
+        table = Table(R)
+        res = {"table_build_ms": table.build_ms}
+        if args.workload == "conj":
+            res["conjunction"] = bench_conj(R, table)
+        else:
+            res["batch"] = bench_batch(R, table, steps=max(1, min(args.steps, 20)))
+    else:
+        res = bench_scan(args, R)
+        if not args.no_extra:
+            # config 3 and config 4 legs; a failure here is reported, not fatal to the line
+            try:
+                table = Table(R)
+                res["extra"]["table_1e8x4_bindex_build_ms"] = table.build_ms
+                res["extra"]["conjunction"] = bench_conj(R, table)
+                res["extra"]["batch"] = bench_batch(R, table)
+                del table
+            except Exception as e:  # noqa: BLE001
+                res["extra"]["error"] = "%s: %s" % (type(e).__name__, e)
     print(json.dumps(res), flush=True)
 
 

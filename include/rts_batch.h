@@ -61,8 +61,13 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *plan, const rts_batch_confi
 /* Plan nq <= max_queries queries (host array) and upload them; returns after the upload. */
 hipError_t rts_batch_plan_set_queries(rts_batch_plan *plan, const rts_query_spec *queries,
                                       int nq, hipStream_t stream);
-/* Evaluate the current queries: out holds nq bitmaps of out_stride_words words each;
- * counts (device, nq entries) may be NULL.  Asynchronous on stream. */
+/* As rts_batch_plan_set_queries with each query's bitmap at outs[q] (device, 16-byte
+ * aligned); launches then take out == NULL. */
+hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *plan, const rts_query_spec *queries,
+                                          BITS *const *outs, int nq, hipStream_t stream);
+/* Evaluate the current queries: out holds nq bitmaps of out_stride_words words each (or is
+ * NULL when the queries carry their own outputs); counts (device, nq entries) may be NULL.
+ * Asynchronous on stream. */
 hipError_t rts_batch_plan_launch(rts_batch_plan *plan, BITS *out, int64_t out_stride_words,
                                  unsigned long long *counts, hipStream_t stream);
 /* As rts_batch_plan_launch through a captured hipGraph (captured on first use and again

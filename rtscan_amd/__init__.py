@@ -105,6 +105,22 @@ SIGNATURES = {
     "rts_batch_plan_free": (None, [_vp]),
     "rts_batch_autotune": (_err, [_vp, _vp, ctypes.c_int64, _vp, _int, _int, ctypes.c_char_p, _vp,
                                   ctypes.POINTER(ctypes.c_float), _vp]),
+    "rts_batch_plan_set_queries_out": (_err, [_vp, _vp, _vp, _int, _vp]),
+    # serving
+    "rts_server_default_config": (None, [_vp]),
+    "rts_server_create": (_err, [ctypes.POINTER(_vp), _vp]),
+    "rts_server_add_table": (_err, [_vp, _vp, _vp, _int, ctypes.c_int64, ctypes.POINTER(_int)]),
+    "rts_server_start": (_err, [_vp]),
+    "rts_server_submit": (_err, [_vp, _int, _vp, ctypes.POINTER(ctypes.c_uint64)]),
+    "rts_server_wait": (_int, [_vp, ctypes.c_uint64, _int, ctypes.POINTER(_vp),
+                               ctypes.POINTER(ctypes.c_ulonglong), ctypes.POINTER(ctypes.c_uint64)]),
+    "rts_server_release": (None, [_vp, ctypes.c_uint64]),
+    "rts_server_stats": (None, [_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64),
+                                ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(_int),
+                                ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
+    "rts_server_stop": (_err, [_vp]),
+    "rts_server_destroy": (None, [_vp]),
+    "rts_server_replay": (_err, [_vp, _vp, _vp, _vp, _int, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
     # generators
     "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
                                    _vp, _int]),
@@ -528,5 +544,106 @@ class BatchPlan:
     def __del__(self):
         try:
             self.free()
+        except Exception:
+            pass
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_int), ("streams", ctypes.c_int),
+                ("pool_bytes", ctypes.c_size_t), ("queue_capacity", ctypes.c_int),
+                ("batch", BatchConfig)]
+
+
+def server_config(**kw):
+    c = ServerConfig()
+    lib().rts_server_default_config(ctypes.byref(c))
+    for k, v in kw.items():
+        if k == "batch":
+            c.batch = v
+        else:
+            setattr(c, k, v)
+    return c
+
+
+class Server:
+    """Request loop over resident tables (include/rts_serve.h)."""
+
+    def __init__(self, config=None):
+        h = _vp()
+        check(lib().rts_server_create(ctypes.byref(h), ctypes.byref(config) if config else None),
+              "rts_server_create")
+        self.h = h.value
+        self.tables = []
+
+    def add_table(self, columns, indexes, n):
+        cols = (_vp * len(columns))(*[c.ptr for c in columns])
+        idx = (_vp * len(columns))(*[(ix.ptr if ix is not None else None) for ix in indexes])
+        tid = _int()
+        check(lib().rts_server_add_table(self.h, cols, idx, len(columns), n, ctypes.byref(tid)),
+              "rts_server_add_table")
+        self.tables.append((columns, indexes, n))
+        return tid.value
+
+    def start(self):
+        check(lib().rts_server_start(self.h), "rts_server_start")
+
+    def submit(self, table, query):
+        arr = query_specs([query])
+        t = ctypes.c_uint64()
+        check(lib().rts_server_submit(self.h, table, arr, ctypes.byref(t)), "rts_server_submit")
+        return t.value
+
+    def wait(self, ticket, timeout_ms=-1):
+        bm = _vp()
+        cnt = ctypes.c_ulonglong()
+        lat = ctypes.c_uint64()
+        st = lib().rts_server_wait(self.h, ticket, timeout_ms, ctypes.byref(bm), ctypes.byref(cnt),
+                                   ctypes.byref(lat))
+        return st, bm.value, cnt.value, lat.value
+
+    def read_bitmap(self, ptr, n):
+        out = np.empty((n + 31) // 32, np.uint32)
+        check(lib().rts_memcpy_dtoh(out.ctypes.data, ptr, out.nbytes), "dtoh")
+        return out
+
+    def release(self, ticket):
+        lib().rts_server_release(self.h, ticket)
+
+    def stats(self):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        m = _int()
+        u, pk = ctypes.c_size_t(), ctypes.c_size_t()
+        lib().rts_server_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c),
+                               ctypes.byref(m), ctypes.byref(u), ctypes.byref(pk))
+        return {"submitted": a.value, "completed": b.value, "batches": c.value,
+                "max_batch_seen": m.value, "pool_in_use": u.value, "pool_peak": pk.value}
+
+    def replay(self, table_ids, queries, arrival_ns):
+        nq = len(queries)
+        tids = np.ascontiguousarray(table_ids, np.int32)
+        arr = query_specs(queries)
+        arrivals = np.ascontiguousarray(arrival_ns, np.uint64)
+        lat = np.zeros(nq, np.uint64)
+        cnt = np.zeros(nq, np.uint64)
+        el = ctypes.c_double()
+        check(lib().rts_server_replay(self.h, tids.ctypes.data, arr, arrivals.ctypes.data, nq,
+                                      lat.ctypes.data, cnt.ctypes.data, ctypes.byref(el)),
+              "rts_server_replay")
+        return lat, cnt, el.value
+
+    def stop(self):
+        check(lib().rts_server_stop(self.h), "rts_server_stop")
+
+    def destroy(self):
+        if getattr(self, "h", None):
+            lib().rts_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
         except Exception:
             pass

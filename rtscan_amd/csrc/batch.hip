@@ -33,37 +33,47 @@ struct BatchBound {
 };
 
 struct BatchCol {
-  BatchBound lo, hi;      // BinDex form: (hi ^ t_hi) & ~(lo ^ t_lo)
-  const CODE *col;        // direct form when non-null: (x - dlo) <= dspan
+  BatchBound lo, hi;  // BinDex form: (fv_hi & ~fv_lo) ^ T, T = refine toggles of both bounds
+  const CODE *col;    // direct form when non-null: T = (x - dlo) <= dspan
   CODE dlo, dspan;
 };
 
 struct BatchQuery {
   int ncols;
   int empty;  // some column's range is empty: all-zero result
+  BITS *out;  // this query's bitmap; nullptr: out + q * out_stride of the launch
   BatchCol c[kBatchMaxCols];
 };
 
+// A work item's query group: queries [q0, q0 + nq), their columns numbered as slots in
+// order; slot s belongs to query q0 + sq[s], column sc[s].
+struct BatchGroup {
+  int q0, nq, nslots, pad;
+  uint8_t sq[kBatchMaxSlots];
+  uint8_t sc[kBatchMaxSlots];
+};
+
+// LDS of one workgroup: one toggle array per slot (t_lo ^ t_hi of its two bounds, or the
+// predicate words of a direct slot) and the flattened list table of the current item.
 template <int TW>
 struct BatchLds {
-  uint32_t tog[kBatchMaxCols][2][TW];
+  static constexpr int kSlots = kBatchLdsWords / TW;
+  uint32_t tog[kSlots][TW];
+  const uint32_t *lpos[2 * kSlots];
+  const CODE *lval[2 * kSlots];
+  uint32_t lstart[2 * kSlots];
+  uint32_t lpref[2 * kSlots + 1];
+  CODE lthr[2 * kSlots];
+  const BITS *fvlo[kSlots];
+  const BITS *fvhi[kSlots];
+  uint32_t lobase[kSlots], hibase[kSlots];
+  uint32_t slot_q[kSlots];
   uint32_t qcount[kBatchMaxGroup];
 };
 
-template <int TW>
-__device__ __forceinline__ void batch_mark(const BatchBound &b, int64_t t, uint32_t *tog) {
-  if (!b.dir) return;
-  const uint32_t e0 = b.dir[t], e1 = b.dir[t + 1];
-  const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += kBlock) {
-    const uint32_t p = b.bpos[e];
-    if (b.bval[e] >= b.thr) atomicOr(&tog[(p - row0) >> 5], 0x80000000u >> (p & 31));
-  }
-}
-
-// Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers
-// TW/4 words, 256 rows (8 words) per step, with the DPP assembly of k_scan_tiles.  Rows
-// past n give 0 bits; the tile's words past the bitmap end are left at 0.
+// Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW/4
+// words, 256 rows (8 words) per step, with the DPP assembly of k_scan_tiles.  Rows past n
+// give 0 bits.
 template <int TW>
 __device__ __forceinline__ void batch_direct(const BatchCol &c, int64_t t, int64_t n,
                                              uint32_t *dst) {
@@ -90,27 +100,48 @@ __device__ __forceinline__ void batch_direct(const BatchCol &c, int64_t t, int64
   }
 }
 
-__device__ __forceinline__ uint4 batch_base(const BatchBound &b, int64_t w, int64_t words) {
-  if (b.base == kBaseZero) return make_uint4(0, 0, 0, 0);
-  if (b.base == kBaseOnes) return make_uint4(~0u, ~0u, ~0u, ~0u);
-  if (w + 3 < words) return *reinterpret_cast<const uint4 *>(b.fv + w);
-  uint4 v = make_uint4(0, 0, 0, 0);
-  v.x = b.fv[w];
-  if (w + 1 < words) v.y = b.fv[w + 1];
-  if (w + 2 < words) v.z = b.fv[w + 2];
-  return v;
+template <int KP>
+struct Words {
+  uint32_t v[KP];
+};
+
+template <int KP>
+__device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, int64_t w,
+                                                int64_t words) {
+  Words<KP> r;
+  if (base != kBaseFv) {
+#pragma unroll
+    for (int k = 0; k < KP; k++) r.v[k] = base == kBaseOnes ? ~0u : 0u;
+    return r;
+  }
+  if (w + KP <= words) {
+#pragma unroll
+    for (int k = 0; k < KP; k += 4) {
+      const uint4 x = *reinterpret_cast<const uint4 *>(fv + w + k);
+      r.v[k] = x.x;
+      r.v[k + 1] = x.y;
+      r.v[k + 2] = x.z;
+      r.v[k + 3] = x.w;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < KP; k++) r.v[k] = w + k < words ? fv[w + k] : 0u;
+  }
+  return r;
 }
 
 template <int TW>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__ qs, int nq,
-                                                  int qgroup, int64_t n, int64_t words,
-                                                  int64_t ntiles, BITS *out,
-                                                  int64_t out_stride, uint32_t *partials) {
+__global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__ qs,
+                                                  const BatchGroup *__restrict__ groups,
+                                                  int ngroups, int64_t n, int64_t words,
+                                                  int64_t ntiles, BITS *out, int64_t out_stride,
+                                                  uint32_t *partials) {
   constexpr int kPer = TW / kBlock;  // output words per thread
   static_assert(kPer == 4 || kPer == 8, "tile words per thread: a multiple of 4");
-  __shared__ BatchLds<TW> lds;
-  for (int i = threadIdx.x; i < kBatchMaxCols * 2 * TW; i += kBlock) (&lds.tog[0][0][0])[i] = 0;
-  const int ngroups = (nq + qgroup - 1) / qgroup;
+  using Lds = BatchLds<TW>;
+  constexpr int kSlots = Lds::kSlots;
+  __shared__ Lds lds;
+  for (int i = threadIdx.x; i < kSlots * TW; i += kBlock) (&lds.tog[0][0])[i] = 0;
   const int xcd = blockIdx.x % kXcds, nblk_x = gridDim.x / kXcds, j = blockIdx.x / kXcds;
   const int64_t tiles_x = ntiles > xcd ? (ntiles - 1 - xcd) / kXcds + 1 : 0;
   const int64_t items_x = tiles_x * ngroups;
@@ -119,80 +150,123 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__
   __syncthreads();
   for (int64_t it = j; it < items_x; it += nblk_x) {
     const int64_t t = xcd + kXcds * (it / ngroups);
-    const int g = (int)(it % ngroups);
-    const int q0 = g * qgroup, q1 = min(nq, q0 + qgroup);
+    const BatchGroup &g = groups[it % ngroups];
+    const int nslots = g.nslots, nlists = 2 * nslots;
+    const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
+
+    // 1. per-slot bases and the tile's list slices
+    if (threadIdx.x < nlists) {
+      const int s = threadIdx.x >> 1, side = threadIdx.x & 1;
+      const BatchCol &bc = qs[g.q0 + g.sq[s]].c[g.sc[s]];
+      const BatchBound &b = side ? bc.hi : bc.lo;
+      uint32_t st = 0, cnt = 0;
+      if (!bc.col && b.dir) {
+        st = b.dir[t];
+        cnt = b.dir[t + 1] - st;
+      }
+      lds.lstart[threadIdx.x] = st;
+      lds.lpref[threadIdx.x + 1] = cnt;
+      lds.lpos[threadIdx.x] = b.bpos;
+      lds.lval[threadIdx.x] = b.bval;
+      lds.lthr[threadIdx.x] = b.thr;
+      if (side) {
+        lds.fvhi[s] = bc.col ? nullptr : b.fv;
+        lds.hibase[s] = bc.col ? kBaseZero : b.base;
+      } else {
+        lds.fvlo[s] = bc.col ? nullptr : b.fv;
+        lds.lobase[s] = bc.col ? kBaseZero : b.base;
+        lds.slot_q[s] = g.sq[s];
+      }
+    }
+    __syncthreads();
+    if (threadIdx.x < kWave) {  // inclusive scan of the list lengths (nlists <= 64)
+      const int l = threadIdx.x;
+      uint32_t v = l < nlists ? lds.lpref[l + 1] : 0;
+#pragma unroll
+      for (int o = 1; o < kWave; o <<= 1) {
+        const uint32_t u = __shfl_up(v, o, kWave);
+        if (l >= o) v += u;
+      }
+      if (l < nlists) lds.lpref[l + 1] = v;
+      if (l == 0) lds.lpref[0] = 0;
+    }
+    __syncthreads();
+
+    // 2. refine toggles of every list of the group, one flattened pass; direct slots
+    const uint32_t total = lds.lpref[nlists];
+    for (uint32_t e = threadIdx.x; e < total; e += kBlock) {
+      int lo = 0, hi = nlists - 1;  // last list with lpref[l] <= e
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (lds.lpref[mid] <= e)
+          lo = mid;
+        else
+          hi = mid - 1;
+      }
+      const uint32_t idx = lds.lstart[lo] + (e - lds.lpref[lo]);
+      const uint32_t p = lds.lpos[lo][idx];
+      if (lds.lval[lo][idx] >= lds.lthr[lo])
+        atomicXor(&lds.tog[lo >> 1][(p - row0) >> 5], 0x80000000u >> (p & 31));
+    }
+    for (int s = 0; s < nslots; s++) {
+      const BatchCol &bc = qs[g.q0 + g.sq[s]].c[g.sc[s]];
+      if (bc.col) batch_direct<TW>(bc, t, n, lds.tog[s]);
+    }
+    __syncthreads();
+
+    // 3. every slot's base words (all loads issued together), then each query's AND
+    const int64_t w = t * TW + lw;
+    Words<kPer> base[kSlots];
+#pragma unroll
+    for (int s = 0; s < kSlots; s++) {
+      if (s < nslots && w < words) {
+        const Words<kPer> h = base_words<kPer>(lds.fvhi[s], lds.hibase[s], w, words);
+        const Words<kPer> l = base_words<kPer>(lds.fvlo[s], lds.lobase[s], w, words);
+#pragma unroll
+        for (int k = 0; k < kPer; k++) base[s].v[k] = h.v[k] & ~l.v[k];
+      }
+    }
     if (threadIdx.x < kBatchMaxGroup) lds.qcount[threadIdx.x] = 0;
-    for (int q = q0; q < q1; q++) {
-      const BatchQuery &qd = qs[q];
-      const int nc = qd.empty ? 0 : qd.ncols;
-      for (int c = 0; c < nc; c++) {
-        if (qd.c[c].col) {
-          batch_direct<TW>(qd.c[c], t, n, lds.tog[c][1]);
-        } else {
-          batch_mark<TW>(qd.c[c].lo, t, lds.tog[c][0]);
-          batch_mark<TW>(qd.c[c].hi, t, lds.tog[c][1]);
+    __syncthreads();
+    for (int qi = 0; qi < g.nq; qi++) {
+      const int q = g.q0 + qi;
+      uint32_t acc[kPer];
+#pragma unroll
+      for (int k = 0; k < kPer; k++) acc[k] = qs[q].empty ? 0u : ~0u;
+#pragma unroll
+      for (int s = 0; s < kSlots; s++) {
+        if (s < nslots && (int)lds.slot_q[s] == qi) {
+#pragma unroll
+          for (int k = 0; k < kPer; k++) acc[k] &= base[s].v[k] ^ lds.tog[s][lw + k];
         }
       }
-      __syncthreads();
-      const int64_t w = t * TW + lw;
       uint32_t pc = 0;
       if (w < words) {
-        uint32_t acc[kPer];
+        BITS *o = (qs[q].out ? qs[q].out : out + (int64_t)q * out_stride) + w;
 #pragma unroll
-        for (int k = 0; k < kPer; k++) acc[k] = qd.empty ? 0u : ~0u;
-        for (int c = 0; c < nc; c++) {
-          const BatchCol &cc = qd.c[c];
-          uint32_t *tl = &lds.tog[c][0][lw], *th = &lds.tog[c][1][lw];
-#pragma unroll
-          for (int k = 0; k < kPer; k += 4) {
-            uint4 lo = make_uint4(0, 0, 0, 0), hi = make_uint4(0, 0, 0, 0);
-            if (!cc.col) {
-              lo = batch_base(cc.lo, w + k, words);
-              hi = batch_base(cc.hi, w + k, words);
-            }
-            const uint32_t l4[4] = {lo.x, lo.y, lo.z, lo.w}, h4[4] = {hi.x, hi.y, hi.z, hi.w};
-#pragma unroll
-            for (int m = 0; m < 4; m++) {
-              acc[k + m] &= (h4[m] ^ th[k + m]) & ~(l4[m] ^ tl[k + m]);
-              th[k + m] = 0;
-              tl[k + m] = 0;
-            }
-          }
+        for (int k = 0; k < kPer; k++) {
+          if (w + k == words - 1) acc[k] &= last_mask;
+          if (w + k >= words) acc[k] = 0;
+          pc += __popc(acc[k]);
         }
-        BITS *o = out + (int64_t)q * out_stride;
+        if (w + kPer <= words) {
 #pragma unroll
-        for (int k = 0; k < kPer; k += 4) {
-          uint32_t v[4];
+          for (int k = 0; k < kPer; k += 4)
+            *reinterpret_cast<uint4 *>(o + k) = make_uint4(acc[k], acc[k + 1], acc[k + 2], acc[k + 3]);
+        } else {
 #pragma unroll
-          for (int m = 0; m < 4; m++) {
-            v[m] = acc[k + m];
-            if (w + k + m == words - 1) v[m] &= last_mask;
-            if (w + k + m >= words) v[m] = 0;
-            pc += __popc(v[m]);
-          }
-          if (w + k + 3 < words) {
-            *reinterpret_cast<uint4 *>(o + w + k) = make_uint4(v[0], v[1], v[2], v[3]);
-          } else {
-#pragma unroll
-            for (int m = 0; m < 4; m++)
-              if (w + k + m < words) o[w + k + m] = v[m];
-          }
+          for (int k = 0; k < kPer; k++)
+            if (w + k < words) o[k] = acc[k];
         }
-      } else {
-        // words past the bitmap: only clear the toggles a direct column may have left
-        for (int c = 0; c < nc; c++)
-#pragma unroll
-          for (int k = 0; k < kPer; k++) {
-            lds.tog[c][0][lw + k] = 0;
-            lds.tog[c][1][lw + k] = 0;
-          }
       }
       pc = wave_sum(pc);
-      if ((threadIdx.x & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[q - q0], pc);
-      __syncthreads();
+      if ((threadIdx.x & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
     }
-    if (threadIdx.x < q1 - q0) partials[(int64_t)(q0 + threadIdx.x) * ntiles + t] = lds.qcount[threadIdx.x];
+    for (int s = 0; s < nslots; s++)
+#pragma unroll
+      for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] = 0;
     __syncthreads();
+    if (threadIdx.x < g.nq) partials[(int64_t)(g.q0 + threadIdx.x) * ntiles + t] = lds.qcount[threadIdx.x];
   }
 }
 
@@ -247,8 +321,11 @@ struct rts_batch_plan {
   rts_batch_config cfg{};
   int64_t ntiles = 0;
   BatchQuery *d_queries = nullptr;
+  BatchGroup *d_groups = nullptr;
+  int ngroups = 0;
   uint32_t *d_partials = nullptr;
   std::vector<BatchQuery> h_queries;
+  std::vector<BatchGroup> h_groups;
   std::vector<uint32_t *> dir_base;  // per column: bdir re-tiled for cfg.tile_words
   std::vector<int64_t> dir_tiles;
   hipGraph_t graph = nullptr;
@@ -259,6 +336,8 @@ struct rts_batch_plan {
   int64_t graph_stride = 0;
   int stats_bindex_cols = 0, stats_direct_cols = 0, stats_narrow_cols = 0;
   std::vector<rts_query_spec> specs;  // the current queries, re-planned on a new tiling
+  std::vector<BITS *> outs;           // their per-query outputs, if given
+  bool has_outs = false;
 };
 
 namespace rts {
@@ -343,12 +422,12 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
   grid = std::max<unsigned>(kXcds, grid / kXcds * kXcds);
   switch (p->cfg.tile_words) {
     case 2048:
-      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->nq,
-                         p->cfg.queries_per_group, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->d_groups,
+                         p->ngroups, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
       break;
     default:
-      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->nq,
-                         p->cfg.queries_per_group, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->d_groups,
+                         p->ngroups, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !counts) return e;
@@ -394,6 +473,7 @@ hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *column
     return set_error(hipErrorInvalidValue, "rts_batch_plan_create: config");
   }
   hipError_t e = hipMalloc((void **)&p->d_queries, (size_t)max_queries * sizeof(BatchQuery));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_groups, (size_t)max_queries * sizeof(BatchGroup));
   if (e == hipSuccess) e = build_dirs(p, stream);
   if (e == hipSuccess)
     e = hipMalloc((void **)&p->d_partials, (size_t)max_queries * p->ntiles * sizeof(uint32_t));
@@ -430,14 +510,27 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *
   if (e != hipSuccess) return set_error(e, "rts_batch_plan_set_config");
   // the planned queries hold directory rows of the old tiling: plan them again
   const std::vector<rts_query_spec> specs = p->specs;
-  return rts_batch_plan_set_queries(p, specs.data(), (int)specs.size(), stream);
+  const std::vector<BITS *> outs = p->outs;
+  return rts_batch_plan_set_queries_out(p, specs.data(), p->has_outs ? outs.data() : nullptr,
+                                        (int)specs.size(), stream);
 }
 
 hipError_t rts_batch_plan_set_queries(rts_batch_plan *p, const rts_query_spec *queries, int nq,
                                       hipStream_t stream) {
+  return rts_batch_plan_set_queries_out(p, queries, nullptr, nq, stream);
+}
+
+hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spec *queries,
+                                          BITS *const *outs, int nq, hipStream_t stream) {
   if (!p || (!queries && nq) || nq < 0 || nq > p->max_queries)
     return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries");
+  for (int q = 0; outs && q < nq; q++)
+    if (!outs[q] || ((uintptr_t)outs[q] & 15))
+      return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: outputs must be 16-byte aligned");
+  p->has_outs = outs != nullptr;
+  p->outs.assign(outs ? outs : (BITS *const *)nullptr, outs ? outs + nq : (BITS *const *)nullptr);
   p->h_queries.assign(nq, BatchQuery{});
+  for (int q = 0; outs && q < nq; q++) p->h_queries[q].out = outs[q];
   p->stats_bindex_cols = p->stats_direct_cols = p->stats_narrow_cols = 0;
   for (int q = 0; q < nq; q++) {
     const rts_query_spec &in = queries[q];
@@ -484,15 +577,40 @@ hipError_t rts_batch_plan_set_queries(rts_batch_plan *p, const rts_query_spec *q
   }
   p->nq = nq;
   p->specs.assign(queries, queries + nq);
+  // Work-item groups: consecutive queries while their columns fit the LDS slots of one tile
+  // and the configured group size.
+  const int slots = kBatchLdsWords / p->cfg.tile_words;
+  p->h_groups.clear();
+  for (int q = 0; q < nq;) {
+    BatchGroup g{};
+    g.q0 = q;
+    while (q < nq && g.nq < p->cfg.queries_per_group) {
+      const int need = p->h_queries[q].empty ? 0 : p->h_queries[q].ncols;
+      if (g.nslots + need > slots) break;
+      for (int c = 0; c < need; c++) {
+        g.sq[g.nslots] = (uint8_t)g.nq;
+        g.sc[g.nslots] = (uint8_t)c;
+        g.nslots++;
+      }
+      g.nq++;
+      q++;
+    }
+    p->h_groups.push_back(g);
+  }
+  p->ngroups = (int)p->h_groups.size();
   if (nq == 0) return hipSuccess;
   hipError_t e = hipMemcpyAsync(p->d_queries, p->h_queries.data(), (size_t)nq * sizeof(BatchQuery),
                                 hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_groups, p->h_groups.data(), (size_t)p->ngroups * sizeof(BatchGroup),
+                       hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);  // h_queries may change on the next call
   return set_error(e, "rts_batch_plan_set_queries");
 }
 
 static bool valid_output(const rts_batch_plan *p, const BITS *out, int64_t stride) {
-  return p && out && stride >= p->words && stride % 4 == 0 && ((uintptr_t)out & 15) == 0;
+  if (p && !out) return p->has_outs;  // per-query outputs given with the queries
+  return p && stride >= p->words && stride % 4 == 0 && ((uintptr_t)out & 15) == 0;
 }
 
 hipError_t rts_batch_plan_launch(rts_batch_plan *p, BITS *out, int64_t out_stride_words,
@@ -552,6 +670,7 @@ void rts_batch_plan_free(rts_batch_plan *p) {
   if (p->graph) (void)hipGraphDestroy(p->graph);
   free_dirs(p);
   (void)hipFree(p->d_queries);
+  (void)hipFree(p->d_groups);
   (void)hipFree(p->d_partials);
   delete p;
 }
