@@ -431,6 +431,87 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
     }
 
 
+def serve_queries(rng, table_cols, nq, ncols_max):
+    """Config-4 query shapes for the serving workload: 1..ncols_max columns chosen uniformly,
+    per-column selectivity log-uniform in [1e-5, 0.9], endpoints from sample quantiles."""
+    qs = []
+    for _ in range(nq):
+        k = int(rng.integers(1, ncols_max + 1))
+        q = []
+        for c in rng.choice(ncols_max, k, replace=False):
+            sel = float(np.exp(rng.uniform(np.log(1e-5), np.log(0.9))))
+            s = table_cols[c]
+            q0 = float(rng.random()) * (1.0 - sel)
+            lo = int(s[int(q0 * (s.size - 1))])
+            hi = int(s[min(s.size - 1, int((q0 + sel) * (s.size - 1)))])
+            q.append((int(c), lo, hi, 0))
+        qs.append(q)
+    return qs
+
+
+def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000_000_000,
+                seed=55, pool_gb=32):
+    """Config 5: two resident tables, T1 = 2^26 rows x 2 uniform columns, T2 = n2 rows x 4
+    columns (uniform, normal, Zipf 1.1, Zipf 1.5), a BinDex per column; Poisson arrivals at
+    each rate, every query on T1 or T2 with probability 1/2, shapes as config 4; admission by
+    the result pool's free HBM; continuous batching over 3 streams."""
+    t_setup = time.time()
+    n1 = 1 << 26
+    t1_cols = [R.generate_column(0, seed, c, n1) for c in range(2)]
+    t1_d = [R.DeviceBuffer.from_numpy(c) for c in t1_cols]
+    t1_ix = [R.BinDexIndex(d, n1) for d in t1_d]
+    t2_d, t2_ix, t2_samples = [], [], []
+    rng = np.random.default_rng(seed)
+    for c, (k, p0, p1) in enumerate(TABLE_KINDS):  # one column at a time bounds host memory
+        col = R.generate_column(k, seed + 1, c, n2, p0, p1)
+        t2_samples.append(np.sort(col[rng.integers(0, n2, 1_000_000)]))
+        t2_d.append(R.DeviceBuffer.from_numpy(col))
+        del col
+        t2_ix.append(R.BinDexIndex(t2_d[-1], n2))
+    t1_samples = [np.sort(c[rng.integers(0, n1, 1_000_000)]) for c in t1_cols]
+    setup_s = time.time() - t_setup
+    cfg = R.server_config(max_batch=256, streams=3, pool_bytes=pool_gb << 30)
+    srv = R.Server(cfg)
+    tid1 = srv.add_table(t1_d, t1_ix, n1)
+    tid2 = srv.add_table(t2_d, t2_ix, n2)
+    srv.start()
+    out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb, "rates": []}
+    for rate, secs in zip(rates, seconds):
+        nq = int(rate * secs)
+        tids = [tid1 if rng.random() < 0.5 else tid2 for _ in range(nq)]
+        qs1 = serve_queries(rng, t1_samples, nq, 2)
+        qs2 = serve_queries(rng, t2_samples, nq, 4)
+        qs = [qs1[i] if tids[i] == tid1 else qs2[i] for i in range(nq)]
+        arrivals = np.cumsum(rng.exponential(1e9 / rate, nq)).astype(np.uint64)
+        st0 = srv.stats()
+        lat, cnt, elapsed = srv.replay(tids, qs, arrivals)
+        st1 = srv.stats()
+        lat_ms = lat.astype(np.float64) / 1e6
+        # bit-exactness: counts of a sample of queries against the one-query-per-launch path
+        check = list(range(0, nq, max(1, nq // 16)))
+        one1 = R.DeviceBuffer(4 * ((n1 + 31) // 32))
+        one2 = R.DeviceBuffer(4 * ((n2 + 31) // 32))
+        c = R.DeviceBuffer(8)
+        ok = True
+        for i in check:
+            q = qs[i]
+            ix = t1_ix if tids[i] == tid1 else t2_ix
+            R.bindex_query([ix[cc] for cc, _, _, _ in q], [lo for _, lo, _, _ in q],
+                           [hi for _, _, hi, _ in q], one1 if tids[i] == tid1 else one2, c)
+            ok = ok and int(c.to_numpy(np.uint64, 1)[0]) == int(cnt[i])
+        out["rates"].append({
+            "offered_qps": rate, "queries": nq,
+            "throughput_qps": nq / elapsed,
+            "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+            "mean_ms": float(lat_ms.mean()),
+            "batches": st1["batches"] - st0["batches"], "max_batch": st1["max_batch_seen"],
+            "pool_peak_gb": st1["pool_peak"] / 2 ** 30,
+            "sampled_counts_match": bool(ok), "checked": len(check),
+        })
+    srv.destroy()
+    return out
+
+
 def bench_copycal(args, R):
     """Calibration for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE: five device-to-device
     copies of exactly 400,000,000 bytes (the HIP runtime's plain copy kernel)."""
@@ -448,7 +529,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="scan", choices=["scan", "copycal", "conj", "batch"])
+    ap.add_argument("--workload", default="scan",
+                    choices=["scan", "copycal", "conj", "batch", "serve"])
+    ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
+                    help="rows of the large serving table")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction and batched legs of the default run")
     args = ap.parse_args()
@@ -464,6 +548,8 @@ def main():
         sys.exit(1)
     if args.workload == "copycal":
         res = bench_copycal(args, R)
+    elif args.workload == "serve":
+        res = {"serve": bench_serve(R, n2=args.serve_rows)}
     elif args.workload in ("conj", "batch"):
         table = Table(R)
         res = {"table_build_ms": table.build_ms}

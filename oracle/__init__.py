@@ -87,6 +87,16 @@ def lib():
                                        ctypes.c_uint32]
         L.orc_bindex_scan.argtypes = [ctypes.POINTER(_OrcBindex), _u32p, ctypes.c_int,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.orc_normal_encode.argtypes = [_u32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                        _u32p, _i32p, _u32p, _u32p, _u32p]
+        L.orc_normal_encode.restype = ctypes.c_int64
+        L.orc_dispersed_map.argtypes = [_u32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
+                                        _u32p, _u32p, _u32p]
+        L.orc_dispersed_map.restype = ctypes.c_int64
+        L.orc_encode_query.argtypes = [_u32p, _u32p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_int,
+                                       ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint32)]
+        L.orc_find_key_by_value.argtypes = [_u32p, _u32p, ctypes.c_int64, ctypes.c_uint32]
+        L.orc_find_key_by_value.restype = ctypes.c_uint32
         for f in ("orc_refine_area", "orc_refine_eq_area"):
             getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int, _u32p,
                                       ctypes.c_int, ctypes.c_int, _u32p]
@@ -314,3 +324,45 @@ def refine_area(bindexes, columns, predicate, sel, inverse, bitmap, eq=False):
     pred = np.ascontiguousarray(predicate, np.uint32)
     f = lib().orc_refine_eq_area if eq else lib().orc_refine_area
     return f(bx, rp, k, pred, sel, int(bool(inverse)), bitmap) == 0
+
+
+def normal_encode(data, encode_min, encode_max):
+    """normalEncode, 7-argument form (remap.cpp:110-203), for one column.  Returns (codes,
+    sorted_pos, sorted_codes, map_values, map_starts)."""
+    data = np.ascontiguousarray(data, np.uint32)
+    n = data.size
+    codes = np.zeros(n, np.uint32)
+    pos = np.zeros(n, np.int32)
+    sc = np.zeros(n, np.uint32)
+    mv = np.zeros(n, np.uint32)
+    ms = np.zeros(n, np.uint32)
+    m = lib().orc_normal_encode(data, n, encode_min, encode_max, codes, pos, sc, mv, ms)
+    return codes, pos, sc, mv[:m].copy(), ms[:m].copy()
+
+
+def dispersed_map(data, encode_min, encode_max):
+    """Encode map of normalEncode's 5-argument form (remap.cpp:205-241): (values, starts,
+    steps)."""
+    data = np.ascontiguousarray(data, np.uint32)
+    n = data.size
+    mv = np.zeros(n, np.uint32)
+    ms = np.zeros(n, np.uint32)
+    st = np.zeros(n, np.uint32)
+    m = lib().orc_dispersed_map(data, n, encode_min, encode_max, mv, ms, st)
+    return mv[:m].copy(), ms[:m].copy(), st[:m].copy()
+
+
+def encode_query(map_values, map_starts, q, op, sentinel):
+    """encodeQuery (remap.cpp:243-272); None for eq / bt."""
+    out = ctypes.c_uint32()
+    rc = lib().orc_encode_query(np.ascontiguousarray(map_values, np.uint32),
+                                np.ascontiguousarray(map_starts, np.uint32), len(map_values),
+                                q & 0xFFFFFFFF, op, sentinel & 0xFFFFFFFF, ctypes.byref(out))
+    return None if rc else out.value
+
+
+def find_key_by_value(map_values, map_starts, val):
+    """findKeyByValue (remap.cpp:274-283)."""
+    return int(lib().orc_find_key_by_value(np.ascontiguousarray(map_values, np.uint32),
+                                           np.ascontiguousarray(map_starts, np.uint32),
+                                           len(map_values), val & 0xFFFFFFFF))

@@ -570,3 +570,111 @@ int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws
                       block_first(b, ar, j) + block_len(b, ar, j), bitmap);
   return 0;
 }
+
+/* ------------------------------------------------------------------ remap surface */
+
+int64_t orc_normal_encode(const uint32_t *data, int n, uint32_t encode_min, uint32_t encode_max,
+                          uint32_t *codes, int32_t *sorted_pos, uint32_t *sorted_codes,
+                          uint32_t *map_values, uint32_t *map_starts) {
+  if (n <= 0) return 0;
+  uint32_t *sval = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+  stable_argsort(data, n, sval, sorted_pos);
+  /* remap.cpp:158: int intervalPerObject = (double(max) - double(min)) / double(n) */
+  const int interval = (int)(((double)encode_max - (double)encode_min) / (double)n);
+  /* remap.cpp:175-196 (setDataPerThread, moveDataByPosPerThread): the i-th sorted row gets
+   * encode_min + i * interval, in 32-bit wrap-around arithmetic */
+  for (int i = 0; i < n; i++) {
+    uint32_t c = encode_min + (uint32_t)i * (uint32_t)interval;
+    sorted_codes[i] = c;
+    codes[sorted_pos[i]] = c;
+  }
+  /* remap.cpp:161-165: every distinct value starts where the previous run ended */
+  int64_t m = 0;
+  for (int i = 0; i < n; i++) {
+    if (i == 0 || sval[i] != sval[i - 1]) {
+      map_values[m] = sval[i];
+      map_starts[m] = sorted_codes[i];
+      m++;
+    }
+  }
+  free(sval);
+  return m;
+}
+
+int64_t orc_dispersed_map(const uint32_t *data, int n, uint32_t encode_min, uint32_t encode_max,
+                          uint32_t *map_values, uint32_t *map_starts, uint32_t *map_steps) {
+  if (n <= 0) return 0;
+  uint32_t *sval = (uint32_t *)malloc((size_t)n * sizeof(uint32_t));
+  int32_t *pos = (int32_t *)malloc((size_t)n * sizeof(int32_t));
+  stable_argsort(data, n, sval, pos);
+  /* remap.cpp:222: double intervalPerObject = (double(max) - double(min)) / double(n) */
+  const double per_row = ((double)encode_max - (double)encode_min) / (double)n;
+  /* remap.cpp:224-228: start begins at encode_min and grows by int(interval * count) */
+  uint32_t start = encode_min;
+  int64_t m = 0;
+  int i = 0;
+  while (i < n) {
+    int j = i;
+    while (j < n && sval[j] == sval[i]) j++;
+    const uint32_t step = (uint32_t)(int64_t)(per_row * (double)(j - i));
+    map_values[m] = sval[i];
+    map_starts[m] = start;
+    map_steps[m] = step;
+    start += step;
+    m++;
+    i = j;
+  }
+  free(sval);
+  free(pos);
+  return m;
+}
+
+static int64_t lower_bound_u32(const uint32_t *a, int64_t m, uint32_t q) {
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] < q) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+static int64_t upper_bound_u32(const uint32_t *a, int64_t m, uint32_t q) {
+  int64_t lo = 0, hi = m;
+  while (lo < hi) {
+    int64_t mid = lo + (hi - lo) / 2;
+    if (a[mid] <= q) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+int orc_encode_query(const uint32_t *map_values, const uint32_t *map_starts, int64_t m, uint32_t q,
+                     int op, uint32_t sentinel, uint32_t *out) {
+  int64_t i;
+  switch (op) {
+    case ORC_LT:
+    case ORC_GE:
+      i = lower_bound_u32(map_values, m, q);
+      *out = i == m ? sentinel : map_starts[i];
+      return 0;
+    case ORC_LE:
+      i = upper_bound_u32(map_values, m, q);
+      *out = i == m ? sentinel : map_starts[i] - 1u;
+      return 0;
+    case ORC_GT:
+      i = upper_bound_u32(map_values, m, q);
+      *out = i == m ? sentinel : map_starts[i];
+      return 0;
+    default:
+      return -1;
+  }
+}
+
+uint32_t orc_find_key_by_value(const uint32_t *map_values, const uint32_t *map_starts, int64_t m,
+                               uint32_t val) {
+  uint32_t last = 0;
+  for (int64_t i = 0; i < m; i++) {
+    if (map_starts[i] > val) return last;
+    last = map_values[i];
+  }
+  return 0;
+}

@@ -91,6 +91,34 @@ int orc_refine_area(const orc_bindex *const *bxs, const uint32_t *const *raws, i
 int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
                        const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap);
 
+/* normalEncode, 7-argument form (remap.cpp:110-203), for one column: rank-order codes
+ * code(row at sorted index i) = encode_min + i * interval (uint32 wrap), interval =
+ * (int)((encode_max - encode_min) / n) in double; the encode map gives every distinct value
+ * the code of its first sorted occurrence.  Ties follow the stable sort (the reference's
+ * parallel argsort leaves their order unspecified).  codes[n]; sorted_pos[n];
+ * sorted_codes[n]; map_values / map_starts receive up to n entries; returns the map size.
+ * parity of the tie order is unpinned; everything else is pinned by the invariants in
+ * tests/test_oracle.py. */
+int64_t orc_normal_encode(const uint32_t *data, int n, uint32_t encode_min, uint32_t encode_max,
+                          uint32_t *codes, int32_t *sorted_pos, uint32_t *sorted_codes,
+                          uint32_t *map_values, uint32_t *map_starts);
+/* normalEncode, 5-argument form (remap.cpp:205-241), its encode map: distinct values, start
+ * codes and each value's code range width step = (int)(D * count), D = (max - min) / n in
+ * double; start = encode_min + sum of earlier steps (uint32 wrap).  Row codes are drawn by
+ * rand() in the reference (remap.cpp:234) and are not restated: a row's code lies in
+ * [start, start + step - 1) of its value.  Returns the map size. */
+int64_t orc_dispersed_map(const uint32_t *data, int n, uint32_t encode_min, uint32_t encode_max,
+                          uint32_t *map_values, uint32_t *map_starts, uint32_t *map_steps);
+/* encodeQuery (remap.cpp:243-272) over a map: op lt/ge -> first value >= q, le/gt -> first
+ * value > q; its start code (le: minus 1), or `sentinel` (the reference returns N) when
+ * there is none.  Returns -1 for eq / bt (the reference exits). */
+int orc_encode_query(const uint32_t *map_values, const uint32_t *map_starts, int64_t m, uint32_t q,
+                     int op, uint32_t sentinel, uint32_t *out);
+/* findKeyByValue (remap.cpp:274-283): the last key whose code is <= val, 0 when val is
+ * below every code or at/above the last one. */
+uint32_t orc_find_key_by_value(const uint32_t *map_values, const uint32_t *map_starts, int64_t m,
+                               uint32_t val);
+
 #ifdef __cplusplus
 }
 #endif

@@ -121,6 +121,13 @@ SIGNATURES = {
     "rts_server_stop": (_err, [_vp]),
     "rts_server_destroy": (None, [_vp]),
     "rts_server_replay": (_err, [_vp, _vp, _vp, _vp, _int, _vp, _vp, ctypes.POINTER(ctypes.c_double)]),
+    # value encoding (remap surface)
+    "rts_encode_column": (_err, [_vp, ctypes.c_int64, _u32, _u32, _vp, _vp, _vp, _vp, _vp,
+                                 ctypes.POINTER(ctypes.c_int64), _vp]),
+    "rts_encode_column_dispersed": (_err, [_vp, ctypes.c_int64, _u32, _u32, _u64, _vp, _vp, _vp,
+                                           ctypes.POINTER(ctypes.c_int64), _vp]),
+    "rts_encode_lookup": (_err, [_vp, _vp, ctypes.c_int64, _u32, _int, _u32, ctypes.POINTER(_u32)]),
+    "rts_encode_key_of": (_u32, [_vp, _vp, ctypes.c_int64, _u32]),
     # generators
     "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
                                    _vp, _int]),
@@ -320,6 +327,52 @@ def generate_column(dist, seed, column_id, n, p0=0.0, p1=0.0, threads=None):
     if rc:
         raise ValueError("rts_generate_column: bad parameters")
     return out
+
+
+def encode_column(col, n, encode_min, encode_max, codes=None, want_sorted=False, stream=None):
+    """Rank encoding of a device column (normalEncode, remap.cpp:110-203).  Returns (codes,
+    map_values, map_starts[, sorted_pos, sorted_codes]) as device buffers / numpy maps; codes
+    may be `col` to encode in place."""
+    codes = codes if codes is not None else DeviceBuffer(4 * n)
+    mv, ms = DeviceBuffer(4 * n), DeviceBuffer(4 * n)
+    sp = DeviceBuffer(4 * n) if want_sorted else None
+    sc = DeviceBuffer(4 * n) if want_sorted else None
+    m = ctypes.c_int64()
+    check(lib().rts_encode_column(col.ptr, n, encode_min, encode_max, codes.ptr,
+                                  sp.ptr if sp else None, sc.ptr if sc else None, mv.ptr, ms.ptr,
+                                  ctypes.byref(m), _sh(stream)), "rts_encode_column")
+    check(lib().rts_stream_synchronize(_sh(stream)), "rts_stream_synchronize")
+    out = (codes, mv.to_numpy(np.uint32, m.value), ms.to_numpy(np.uint32, m.value))
+    return out + ((sp, sc) if want_sorted else ())
+
+
+def encode_column_dispersed(col, n, encode_min, encode_max, seed, codes=None, stream=None):
+    """Dispersed encoding (normalEncode 5-argument form, remap.cpp:205-241): (codes,
+    map_values, map_starts)."""
+    codes = codes if codes is not None else DeviceBuffer(4 * n)
+    mv, ms = DeviceBuffer(4 * n), DeviceBuffer(4 * n)
+    m = ctypes.c_int64()
+    check(lib().rts_encode_column_dispersed(col.ptr, n, encode_min, encode_max, seed, codes.ptr,
+                                            mv.ptr, ms.ptr, ctypes.byref(m), _sh(stream)),
+          "rts_encode_column_dispersed")
+    return codes, mv.to_numpy(np.uint32, m.value), ms.to_numpy(np.uint32, m.value)
+
+
+def encode_lookup(map_values, map_starts, old_query, op, sentinel):
+    """encodeQuery (remap.cpp:243-272) over a host map; raises HipError for eq / bt."""
+    mv = np.ascontiguousarray(map_values, np.uint32)
+    ms = np.ascontiguousarray(map_starts, np.uint32)
+    out = _u32()
+    check(lib().rts_encode_lookup(mv.ctypes.data, ms.ctypes.data, mv.size, old_query & 0xFFFFFFFF,
+                                  op, sentinel & 0xFFFFFFFF, ctypes.byref(out)), "rts_encode_lookup")
+    return out.value
+
+
+def encode_key_of(map_values, map_starts, code):
+    """findKeyByValue (remap.cpp:274-283) over a host map."""
+    mv = np.ascontiguousarray(map_values, np.uint32)
+    ms = np.ascontiguousarray(map_starts, np.uint32)
+    return int(lib().rts_encode_key_of(mv.ctypes.data, ms.ctypes.data, mv.size, code & 0xFFFFFFFF))
 
 
 class BinDexIndex:

@@ -19,6 +19,9 @@ def declared_functions():
     names = set()
     for path in glob.glob(os.path.join(ROOT, "include", "*.h")):
         src = open(path).read()
+        # C++-only inline wrappers over the C ABI are not library symbols
+        src = re.sub(r"#if defined\(__cplusplus\) && !defined\(\w+\).*?#endif /\* __cplusplus[^\n]*",
+                     " ", src, flags=re.S)
         src = re.sub(r"/\*.*?\*/", " ", src, flags=re.S)
         src = re.sub(r"//[^\n]*", " ", src)
         src = "\n".join(l for l in src.splitlines()
