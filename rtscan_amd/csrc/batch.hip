@@ -1,9 +1,10 @@
 // batch.hip -- batched evaluation of many range queries in one persistent launch.
 //
 // A query is a conjunction of closed ranges over 1..4 columns of a table (all columns
-// have n rows).  Each query-column is evaluated either from the column's BinDex (filter
-// vectors + the tile slices of two bins' refine lists, as in k_bindex_query) or directly
-// from the column's values (the wave64 DPP word assembly of k_scan_tiles).
+// have n rows).  Each query-column ("slot") is evaluated either from the column's BinDex
+// (two filter vectors and the tile slices of up to two bins' refine lists, as in
+// k_bindex_query) or directly from the column's values (the wave64 DPP word assembly of
+// k_scan_tiles).
 //
 // Work item = (row tile t, query group g): the workgroup writes the tile's output words of
 // every query of the group.  Items are assigned statically so that the items of one tile
@@ -11,6 +12,12 @@
 // takes tiles t with t % 8 == x, tile-major): the tile's filter-vector and list slices are
 // read from HBM once and re-read by the other query groups from that XCD's L2.  Popcounts
 // go to one partial per (query, tile), summed by a second small kernel.
+//
+// The item is latency-bound unless many loads are in flight, so each phase issues its
+// loads together: the group's slot and list descriptors are flattened at plan time (one
+// load each, then the two directory words), refine entries are taken 8 per thread with
+// all 16 loads issued before the LDS toggles, and every slot's filter-vector words are
+// loaded for 4 slots at a time.
 #include <string.h>
 
 #include <algorithm>
@@ -23,38 +30,47 @@ namespace rts {
 
 enum : uint32_t { kBaseZero = 0, kBaseOnes = 1, kBaseFv = 2 };
 
-struct BatchBound {
-  const BITS *fv;       // kBaseFv
-  const uint32_t *dir;  // tile directory row of the bin, nullptr: no refine list
+constexpr int kBatchThreads = 512;  // 8 waves per workgroup, 2 workgroups per CU (LDS)
+constexpr int kBatchWaves = kBatchThreads / kWave;
+constexpr int kEntryUnroll = 8;     // refine entries per thread per round
+
+// One query-column of a group.  BinDex form: (fv_hi & ~fv_lo) ^ T, with T the toggles of
+// the slot's refine lists; bases kBaseZero / kBaseOnes need no filter vector.  Direct form
+// (col non-null): the words of (x - dlo) <= dspan.
+struct BatchSlot {
+  const BITS *fvhi;
+  const BITS *fvlo;
+  const CODE *col;
+  CODE dlo, dspan;
+  uint32_t hibase, lobase;
+};
+
+// One refine list of a group: the rows of a bin's B list (in row order, tile directory
+// `dir`) with value >= thr flip bits of slot `slot`.
+struct BatchList {
+  const uint32_t *dir;
   const uint32_t *bpos;
   const CODE *bval;
   CODE thr;
-  uint32_t base;
+  uint32_t slot;
 };
 
-struct BatchCol {
-  BatchBound lo, hi;  // BinDex form: (fv_hi & ~fv_lo) ^ T, T = refine toggles of both bounds
-  const CODE *col;    // direct form when non-null: T = (x - dlo) <= dspan
-  CODE dlo, dspan;
-};
-
+// Output of one query: its own bitmap, or out + q * out_stride of the launch.
 struct BatchQuery {
-  int ncols;
+  BITS *out;
   int empty;  // some column's range is empty: all-zero result
-  BITS *out;  // this query's bitmap; nullptr: out + q * out_stride of the launch
-  BatchCol c[kBatchMaxCols];
+  int pad;
 };
 
-// A work item's query group: queries [q0, q0 + nq), their columns numbered as slots in
-// order; slot s belongs to query q0 + sq[s], column sc[s].
+// A work item's query group: queries [q0, q0 + nq) with slots [slot0, slot0 + nslots) and
+// lists [list0, list0 + nlists) of the plan; query i owns slots [qslot[i], qslot[i + 1]).
 struct BatchGroup {
-  int q0, nq, nslots, pad;
-  uint8_t sq[kBatchMaxSlots];
-  uint8_t sc[kBatchMaxSlots];
+  int q0, nq, nslots, nlists;
+  int slot0, list0;
+  uint8_t qslot[kBatchMaxGroup + 1];
 };
 
-// LDS of one workgroup: one toggle array per slot (t_lo ^ t_hi of its two bounds, or the
-// predicate words of a direct slot) and the flattened list table of the current item.
+// LDS of one workgroup: one toggle array per slot and the current item's descriptors.
 template <int TW>
 struct BatchLds {
   static constexpr int kSlots = kBatchLdsWords / TW;
@@ -64,39 +80,53 @@ struct BatchLds {
   uint32_t lstart[2 * kSlots];
   uint32_t lpref[2 * kSlots + 1];
   CODE lthr[2 * kSlots];
-  const BITS *fvlo[kSlots];
-  const BITS *fvhi[kSlots];
-  uint32_t lobase[kSlots], hibase[kSlots];
-  uint32_t slot_q[kSlots];
+  uint32_t lslot[2 * kSlots];
+  BatchSlot slot[kSlots];
+  BITS *qout[kBatchMaxGroup];
+  uint32_t qempty[kBatchMaxGroup];
+  uint32_t qslot[kBatchMaxGroup + 1];
   uint32_t qcount[kBatchMaxGroup];
 };
 
-// Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW/4
-// words, 256 rows (8 words) per step, with the DPP assembly of k_scan_tiles.  Rows past n
-// give 0 bits.
+// Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW / 8
+// words, 256 rows (8 words) per step, 4 steps' loads in flight, with the DPP assembly of
+// k_scan_tiles.  Rows past n give 0 bits.
 template <int TW>
-__device__ __forceinline__ void batch_direct(const BatchCol &c, int64_t t, int64_t n,
-                                             uint32_t *dst) {
+__device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int64_t n, uint32_t *dst) {
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   const uint32_t shift = 28 - 4 * (lane & 7);
-  constexpr int kWordsPerWave = TW / (kBlock / kWave);
+  constexpr int kWordsPerWave = TW / kBatchWaves;
+  constexpr int kSteps = kWordsPerWave / 8;
+  constexpr int kU = 4;
+  static_assert(kSteps % kU == 0, "steps per wave");
   const int64_t tile_row0 = t * (int64_t)TW * 32;
-  for (int s = 0; s < kWordsPerWave / 8; s++) {
-    const int w_local = wv * kWordsPerWave + s * 8;  // first word of this 256-row step
-    const int64_t r0 = tile_row0 + (int64_t)w_local * 32;
-    uint32_t nib = 0;
-    if (r0 + 256 <= n) {
-      const uint4 v = load_stream(reinterpret_cast<const uint4 *>(c.col + r0) + lane);
-      nib = nibble4(v, c.dlo, c.dspan);
-    } else {
+  const int64_t wave_row0 = tile_row0 + (int64_t)wv * kWordsPerWave * 32;
+  if (wave_row0 + (int64_t)kWordsPerWave * 32 <= n) {
+    const uint4 *src = reinterpret_cast<const uint4 *>(c.col + wave_row0) + lane;
+#pragma unroll 1
+    for (int s0 = 0; s0 < kSteps; s0 += kU) {
+      uint4 v[kU];
 #pragma unroll
-      for (int j = 0; j < 4; j++) {
-        const int64_t r = r0 + 4 * lane + j;
-        if (r < n && (c.col[r] - c.dlo) <= c.dspan) nib |= 8u >> j;
+      for (int u = 0; u < kU; u++) v[u] = load_stream(src + (s0 + u) * kWave);
+#pragma unroll
+      for (int u = 0; u < kU; u++) {
+        const uint32_t w = or_lanes8(nibble4(v[u], c.dlo, c.dspan) << shift);
+        if ((lane & 7) == 0) dst[wv * kWordsPerWave + (s0 + u) * 8 + lane / 8] = w;
       }
     }
+    return;
+  }
+#pragma unroll 1
+  for (int s = 0; s < kSteps; s++) {  // the tile holding row n - 1
+    const int64_t r0 = wave_row0 + (int64_t)s * 256;
+    uint32_t nib = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int64_t r = r0 + 4 * lane + j;
+      if (r < n && (c.col[r] - c.dlo) <= c.dspan) nib |= 8u >> j;
+    }
     const uint32_t w = or_lanes8(nib << shift);
-    if ((lane & 7) == 0) dst[w_local + lane / 8] = w;
+    if ((lane & 7) == 0) dst[wv * kWordsPerWave + s * 8 + lane / 8] = w;
   }
 }
 
@@ -106,8 +136,7 @@ struct Words {
 };
 
 template <int KP>
-__device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, int64_t w,
-                                                int64_t words) {
+__device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, int64_t w, int64_t words) {
   Words<KP> r;
   if (base != kBaseFv) {
 #pragma unroll
@@ -115,13 +144,19 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
     return r;
   }
   if (w + KP <= words) {
+    if constexpr (KP == 2) {
+      const uint2 x = *reinterpret_cast<const uint2 *>(fv + w);
+      r.v[0] = x.x;
+      r.v[1] = x.y;
+    } else {
 #pragma unroll
-    for (int k = 0; k < KP; k += 4) {
-      const uint4 x = *reinterpret_cast<const uint4 *>(fv + w + k);
-      r.v[k] = x.x;
-      r.v[k + 1] = x.y;
-      r.v[k + 2] = x.z;
-      r.v[k + 3] = x.w;
+      for (int k = 0; k < KP; k += 4) {
+        const uint4 x = *reinterpret_cast<const uint4 *>(fv + w + k);
+        r.v[k] = x.x;
+        r.v[k + 1] = x.y;
+        r.v[k + 2] = x.z;
+        r.v[k + 3] = x.w;
+      }
     }
   } else {
 #pragma unroll
@@ -130,119 +165,150 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
   return r;
 }
 
+template <int KP>
+__device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
+  if constexpr (KP == 2) {
+    *reinterpret_cast<uint2 *>(o) = make_uint2(acc[0], acc[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < KP; k += 4)
+      *reinterpret_cast<uint4 *>(o + k) = make_uint4(acc[k], acc[k + 1], acc[k + 2], acc[k + 3]);
+  }
+}
+
 template <int TW>
-__global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__ qs,
-                                                  const BatchGroup *__restrict__ groups,
-                                                  int ngroups, int64_t n, int64_t words,
-                                                  int64_t ntiles, BITS *out, int64_t out_stride,
-                                                  uint32_t *partials) {
-  constexpr int kPer = TW / kBlock;  // output words per thread
-  static_assert(kPer == 4 || kPer == 8, "tile words per thread: a multiple of 4");
+__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
+    const BatchGroup *__restrict__ groups, const BatchSlot *__restrict__ slots,
+    const BatchList *__restrict__ lists, const BatchQuery *__restrict__ queries,
+    const int *__restrict__ ngroups_ptr, int64_t n, int64_t words, int64_t ntiles, BITS *out,
+    int64_t out_stride, uint32_t *partials) {
+  constexpr int kPer = TW / kBatchThreads;  // output words per thread
+  static_assert(kPer == 2 || kPer == 4, "tile words per thread");
   using Lds = BatchLds<TW>;
   constexpr int kSlots = Lds::kSlots;
   __shared__ Lds lds;
-  for (int i = threadIdx.x; i < kSlots * TW; i += kBlock) (&lds.tog[0][0])[i] = 0;
+  const int tid = threadIdx.x;
+  for (int i = tid; i < kSlots * TW; i += kBatchThreads) (&lds.tog[0][0])[i] = 0;
+  if (tid < kBatchMaxGroup) lds.qcount[tid] = 0;
+  // the group count is read from device memory, so a captured launch stays valid when the
+  // queries are planned again
+  const int ngroups = *ngroups_ptr;
   const int xcd = blockIdx.x % kXcds, nblk_x = gridDim.x / kXcds, j = blockIdx.x / kXcds;
   const int64_t tiles_x = ntiles > xcd ? (ntiles - 1 - xcd) / kXcds + 1 : 0;
   const int64_t items_x = tiles_x * ngroups;
   const uint32_t last_mask = tail_mask((uint64_t)n);
-  const int lw = threadIdx.x * kPer;
+  const int lw = tid * kPer;
   __syncthreads();
   for (int64_t it = j; it < items_x; it += nblk_x) {
     const int64_t t = xcd + kXcds * (it / ngroups);
     const BatchGroup &g = groups[it % ngroups];
-    const int nslots = g.nslots, nlists = 2 * nslots;
+    const int nslots = g.nslots, nlists = g.nlists, nq = g.nq, q0 = g.q0;
     const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
 
-    // 1. per-slot bases and the tile's list slices
-    if (threadIdx.x < nlists) {
-      const int s = threadIdx.x >> 1, side = threadIdx.x & 1;
-      const BatchCol &bc = qs[g.q0 + g.sq[s]].c[g.sc[s]];
-      const BatchBound &b = side ? bc.hi : bc.lo;
-      uint32_t st = 0, cnt = 0;
-      if (!bc.col && b.dir) {
-        st = b.dir[t];
-        cnt = b.dir[t + 1] - st;
-      }
-      lds.lstart[threadIdx.x] = st;
-      lds.lpref[threadIdx.x + 1] = cnt;
-      lds.lpos[threadIdx.x] = b.bpos;
-      lds.lval[threadIdx.x] = b.bval;
-      lds.lthr[threadIdx.x] = b.thr;
-      if (side) {
-        lds.fvhi[s] = bc.col ? nullptr : b.fv;
-        lds.hibase[s] = bc.col ? kBaseZero : b.base;
-      } else {
-        lds.fvlo[s] = bc.col ? nullptr : b.fv;
-        lds.lobase[s] = bc.col ? kBaseZero : b.base;
-        lds.slot_q[s] = g.sq[s];
-      }
+    // 1. descriptors of the item into LDS: lists (and their tile slices), slots, queries
+    if (tid < nlists) {
+      const BatchList L = lists[g.list0 + tid];
+      const uint32_t st = L.dir[t];
+      lds.lstart[tid] = st;
+      lds.lpref[tid + 1] = L.dir[t + 1] - st;
+      lds.lpos[tid] = L.bpos;
+      lds.lval[tid] = L.bval;
+      lds.lthr[tid] = L.thr;
+      lds.lslot[tid] = L.slot;
+    } else if (tid >= kWave && tid < kWave + nslots) {
+      lds.slot[tid - kWave] = slots[g.slot0 + tid - kWave];
+    } else if (tid >= 2 * kWave && tid < 2 * kWave + nq) {
+      const int qi = tid - 2 * kWave;
+      const BatchQuery Q = queries[q0 + qi];
+      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)(q0 + qi) * out_stride;
+      lds.qempty[qi] = Q.empty;
     }
+    if (tid >= 3 * kWave && tid <= 3 * kWave + nq) lds.qslot[tid - 3 * kWave] = g.qslot[tid - 3 * kWave];
     __syncthreads();
-    if (threadIdx.x < kWave) {  // inclusive scan of the list lengths (nlists <= 64)
-      const int l = threadIdx.x;
-      uint32_t v = l < nlists ? lds.lpref[l + 1] : 0;
+    if (tid < kWave) {  // inclusive scan of the list lengths (nlists <= 32)
+      uint32_t v = tid < nlists ? lds.lpref[tid + 1] : 0;
 #pragma unroll
       for (int o = 1; o < kWave; o <<= 1) {
         const uint32_t u = __shfl_up(v, o, kWave);
-        if (l >= o) v += u;
+        if (tid >= o) v += u;
       }
-      if (l < nlists) lds.lpref[l + 1] = v;
-      if (l == 0) lds.lpref[0] = 0;
+      if (tid < nlists) lds.lpref[tid + 1] = v;
+      if (tid == 0) lds.lpref[0] = 0;
     }
     __syncthreads();
 
-    // 2. refine toggles of every list of the group, one flattened pass; direct slots
+    // 2. refine toggles of every list, kEntryUnroll entries per thread with their loads
+    //    issued together; direct slots write their predicate words
     const uint32_t total = lds.lpref[nlists];
-    for (uint32_t e = threadIdx.x; e < total; e += kBlock) {
-      int lo = 0, hi = nlists - 1;  // last list with lpref[l] <= e
-      while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (lds.lpref[mid] <= e)
-          lo = mid;
-        else
-          hi = mid - 1;
+    for (uint32_t e0 = 0; e0 < total; e0 += kBatchThreads * kEntryUnroll) {
+      uint32_t pos[kEntryUnroll], val[kEntryUnroll], lid[kEntryUnroll];
+#pragma unroll
+      for (int u = 0; u < kEntryUnroll; u++) {
+        const uint32_t e = e0 + u * kBatchThreads + tid;
+        lid[u] = 0xFFFFFFFFu;
+        if (e < total) {
+          int lo = 0, hi = nlists - 1;  // last list with lpref[l] <= e
+          while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (lds.lpref[mid] <= e)
+              lo = mid;
+            else
+              hi = mid - 1;
+          }
+          const uint32_t idx = lds.lstart[lo] + (e - lds.lpref[lo]);
+          pos[u] = lds.lpos[lo][idx];
+          val[u] = lds.lval[lo][idx];
+          lid[u] = (uint32_t)lo;
+        }
       }
-      const uint32_t idx = lds.lstart[lo] + (e - lds.lpref[lo]);
-      const uint32_t p = lds.lpos[lo][idx];
-      if (lds.lval[lo][idx] >= lds.lthr[lo])
-        atomicXor(&lds.tog[lo >> 1][(p - row0) >> 5], 0x80000000u >> (p & 31));
+#pragma unroll
+      for (int u = 0; u < kEntryUnroll; u++) {
+        if (lid[u] != 0xFFFFFFFFu && val[u] >= lds.lthr[lid[u]]) {
+          const uint32_t p = pos[u];
+          atomicXor(&lds.tog[lds.lslot[lid[u]]][(p - row0) >> 5], 0x80000000u >> (p & 31));
+        }
+      }
     }
-    for (int s = 0; s < nslots; s++) {
-      const BatchCol &bc = qs[g.q0 + g.sq[s]].c[g.sc[s]];
-      if (bc.col) batch_direct<TW>(bc, t, n, lds.tog[s]);
-    }
+    for (int s = 0; s < nslots; s++)
+      if (lds.slot[s].col) batch_direct<TW>(lds.slot[s], t, n, lds.tog[s]);
     __syncthreads();
 
-    // 3. every slot's base words (all loads issued together), then each query's AND
+    // 3a. each BinDex slot's words: tog ^= fv_hi & ~fv_lo, 4 slots' loads in flight.  Every
+    //     thread touches only its own words from here on.
     const int64_t w = t * TW + lw;
-    Words<kPer> base[kSlots];
+    for (int s0 = 0; s0 < nslots; s0 += 4) {
+      Words<kPer> h[4], l[4];
 #pragma unroll
-    for (int s = 0; s < kSlots; s++) {
-      if (s < nslots && w < words) {
-        const Words<kPer> h = base_words<kPer>(lds.fvhi[s], lds.hibase[s], w, words);
-        const Words<kPer> l = base_words<kPer>(lds.fvlo[s], lds.lobase[s], w, words);
+      for (int u = 0; u < 4; u++) {
+        const int s = s0 + u;
+        if (s < nslots && !lds.slot[s].col && w < words) {
+          h[u] = base_words<kPer>(lds.slot[s].fvhi, lds.slot[s].hibase, w, words);
+          l[u] = base_words<kPer>(lds.slot[s].fvlo, lds.slot[s].lobase, w, words);
+        }
+      }
 #pragma unroll
-        for (int k = 0; k < kPer; k++) base[s].v[k] = h.v[k] & ~l.v[k];
+      for (int u = 0; u < 4; u++) {
+        const int s = s0 + u;
+        if (s < nslots && !lds.slot[s].col && w < words) {
+#pragma unroll
+          for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] ^= h[u].v[k] & ~l[u].v[k];
+        }
       }
     }
-    if (threadIdx.x < kBatchMaxGroup) lds.qcount[threadIdx.x] = 0;
-    __syncthreads();
-    for (int qi = 0; qi < g.nq; qi++) {
-      const int q = g.q0 + qi;
+
+    // 3b. each query: AND of its slots, stored with its popcount
+    for (int qi = 0; qi < nq; qi++) {
       uint32_t acc[kPer];
+      const uint32_t fill = lds.qempty[qi] ? 0u : ~0u;
 #pragma unroll
-      for (int k = 0; k < kPer; k++) acc[k] = qs[q].empty ? 0u : ~0u;
+      for (int k = 0; k < kPer; k++) acc[k] = fill;
+      for (uint32_t s = lds.qslot[qi]; s < lds.qslot[qi + 1]; s++) {
 #pragma unroll
-      for (int s = 0; s < kSlots; s++) {
-        if (s < nslots && (int)lds.slot_q[s] == qi) {
-#pragma unroll
-          for (int k = 0; k < kPer; k++) acc[k] &= base[s].v[k] ^ lds.tog[s][lw + k];
-        }
+        for (int k = 0; k < kPer; k++) acc[k] &= lds.tog[s][lw + k];
       }
       uint32_t pc = 0;
       if (w < words) {
-        BITS *o = (qs[q].out ? qs[q].out : out + (int64_t)q * out_stride) + w;
+        BITS *o = lds.qout[qi] + w;
 #pragma unroll
         for (int k = 0; k < kPer; k++) {
           if (w + k == words - 1) acc[k] &= last_mask;
@@ -250,9 +316,7 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__
           pc += __popc(acc[k]);
         }
         if (w + kPer <= words) {
-#pragma unroll
-          for (int k = 0; k < kPer; k += 4)
-            *reinterpret_cast<uint4 *>(o + k) = make_uint4(acc[k], acc[k + 1], acc[k + 2], acc[k + 3]);
+          store_words<kPer>(o, acc);
         } else {
 #pragma unroll
           for (int k = 0; k < kPer; k++)
@@ -260,13 +324,16 @@ __global__ __launch_bounds__(kBlock) void k_batch(const BatchQuery *__restrict__
         }
       }
       pc = wave_sum(pc);
-      if ((threadIdx.x & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
+      if ((tid & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
     }
     for (int s = 0; s < nslots; s++)
 #pragma unroll
       for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] = 0;
     __syncthreads();
-    if (threadIdx.x < g.nq) partials[(int64_t)(g.q0 + threadIdx.x) * ntiles + t] = lds.qcount[threadIdx.x];
+    if (tid < nq) {  // read and cleared by the same thread: the next item starts at 0
+      partials[(int64_t)(q0 + tid) * ntiles + t] = lds.qcount[tid];
+      lds.qcount[tid] = 0;
+    }
   }
 }
 
@@ -288,8 +355,23 @@ __global__ __launch_bounds__(kBlock) void k_batch_counts(const uint32_t *partial
   }
 }
 
-static BatchBound make_batch_bound(const rts_bindex_dev *d, const BoundSpec &s) {
-  BatchBound b{};
+// Planned form of one query-column before grouping.
+struct PlannedBound {
+  uint32_t base = kBaseZero;
+  const BITS *fv = nullptr;
+  int list = -1;  // bin of the refine list, -1: none
+  CODE thr = 0;
+};
+
+struct PlannedCol {
+  int col = 0;
+  bool direct = false;
+  CODE dlo = 0, dspan = 0;
+  PlannedBound lo, hi;
+};
+
+static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
+  PlannedBound b;
   if (s.fv < 0) {
     b.base = kBaseZero;
   } else if (s.fv >= kBins - 1) {
@@ -298,11 +380,8 @@ static BatchBound make_batch_bound(const rts_bindex_dev *d, const BoundSpec &s) 
     b.base = kBaseFv;
     b.fv = d->fvs + (size_t)s.fv * d->fv_stride;
   }
-  if (s.list >= 0) {
-    b.bpos = d->bpos;
-    b.bval = d->bval;
-    b.thr = s.thr;
-  }
+  b.list = s.list;
+  b.thr = s.thr;
   return b;
 }
 
@@ -322,10 +401,15 @@ struct rts_batch_plan {
   int64_t ntiles = 0;
   BatchQuery *d_queries = nullptr;
   BatchGroup *d_groups = nullptr;
+  BatchSlot *d_slots = nullptr;  // max_queries * kBatchMaxCols
+  BatchList *d_lists = nullptr;  // max_queries * kBatchMaxCols * 2
+  int *d_ngroups = nullptr;      // device copy of ngroups
   int ngroups = 0;
   uint32_t *d_partials = nullptr;
   std::vector<BatchQuery> h_queries;
   std::vector<BatchGroup> h_groups;
+  std::vector<BatchSlot> h_slots;
+  std::vector<BatchList> h_lists;
   std::vector<uint32_t *> dir_base;  // per column: bdir re-tiled for cfg.tile_words
   std::vector<int64_t> dir_tiles;
   hipGraph_t graph = nullptr;
@@ -405,7 +489,7 @@ static int batch_resident_per_cu() {
   static int cached = 0;
   if (!cached) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<TW>, kBlock, 0) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<TW>, kBatchThreads, 0) != hipSuccess || nb < 1)
       nb = 1;
     cached = nb;
   }
@@ -422,12 +506,14 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
   grid = std::max<unsigned>(kXcds, grid / kXcds * kXcds);
   switch (p->cfg.tile_words) {
     case 2048:
-      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->d_groups,
-                         p->ngroups, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_slots,
+                         p->d_lists, p->d_queries, p->d_ngroups, p->n, p->words, p->ntiles, out, out_stride,
+                         p->d_partials);
       break;
     default:
-      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBlock), 0, s, p->d_queries, p->d_groups,
-                         p->ngroups, p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_slots,
+                         p->d_lists, p->d_queries, p->d_ngroups, p->n, p->words, p->ntiles, out, out_stride,
+                         p->d_partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !counts) return e;
@@ -474,6 +560,11 @@ hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *column
   }
   hipError_t e = hipMalloc((void **)&p->d_queries, (size_t)max_queries * sizeof(BatchQuery));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_groups, (size_t)max_queries * sizeof(BatchGroup));
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&p->d_slots, (size_t)max_queries * kBatchMaxCols * sizeof(BatchSlot));
+  if (e == hipSuccess)
+    e = hipMalloc((void **)&p->d_lists, (size_t)max_queries * kBatchMaxCols * 2 * sizeof(BatchList));
+  if (e == hipSuccess) e = hipMalloc((void **)&p->d_ngroups, sizeof(int));
   if (e == hipSuccess) e = build_dirs(p, stream);
   if (e == hipSuccess)
     e = hipMalloc((void **)&p->d_partials, (size_t)max_queries * p->ntiles * sizeof(uint32_t));
@@ -527,76 +618,113 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   for (int q = 0; outs && q < nq; q++)
     if (!outs[q] || ((uintptr_t)outs[q] & 15))
       return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: outputs must be 16-byte aligned");
-  p->has_outs = outs != nullptr;
-  p->outs.assign(outs ? outs : (BITS *const *)nullptr, outs ? outs + nq : (BITS *const *)nullptr);
-  p->h_queries.assign(nq, BatchQuery{});
-  for (int q = 0; outs && q < nq; q++) p->h_queries[q].out = outs[q];
-  p->stats_bindex_cols = p->stats_direct_cols = p->stats_narrow_cols = 0;
+  // 1. every query-column in BinDex or direct form
+  std::vector<PlannedCol> pc((size_t)nq * kBatchMaxCols);
+  std::vector<int> qcols(nq, 0), qempty(nq, 0);
+  int st_bindex = 0, st_direct = 0, st_narrow = 0;
   for (int q = 0; q < nq; q++) {
     const rts_query_spec &in = queries[q];
-    BatchQuery &bq = p->h_queries[q];
     if (in.ncols < 1 || in.ncols > kBatchMaxCols)
       return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: ncols");
-    bq.ncols = in.ncols;
+    qcols[q] = in.ncols;
     for (int c = 0; c < in.ncols; c++) {
       const int col = in.column[c];
       if (col < 0 || col >= p->ncols) return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: column");
       const CODE lo = in.lo[c], hi = in.hi[c];
-      if (lo > hi) bq.empty = 1;
-      BatchCol &bc = bq.c[c];
+      if (lo > hi) qempty[q] = 1;
+      PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
+      P.col = col;
       const BinDex *b = p->index[col];
-      const bool direct = in.mode[c] == RTS_EVAL_DIRECT || !b;
       if (in.mode[c] == RTS_EVAL_BINDEX && !b)
         return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: column has no index");
-      if (direct) {
+      if (in.mode[c] == RTS_EVAL_DIRECT || !b) {
         if (((uintptr_t)p->columns[col] & 15) != 0)
           return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: direct column not 16-byte aligned");
-        bc.col = p->columns[col];
-        bc.dlo = lo;
-        bc.dspan = hi - lo;
-        p->stats_direct_cols++;
+        P.direct = true;
+        P.dlo = lo;
+        P.dspan = hi - lo;
+        st_direct++;
         continue;
       }
       const rts_bindex_dev *d = b->dev;
       const BoundSpec sl = lo == 0 ? BoundSpec{-1, -1, 0} : bound_spec(d, lo, false);
       const BoundSpec sh = bound_spec(d, hi + 1u, hi == 0xFFFFFFFFu);
-      bc.lo = make_batch_bound(d, sl);
-      bc.hi = make_batch_bound(d, sh);
-      const int64_t per = p->ntiles + 1;
-      if (sl.list >= 0) bc.lo.dir = p->dir_base[col] + (size_t)sl.list * per;
-      if (sh.list >= 0) bc.hi.dir = p->dir_base[col] + (size_t)sh.list * per;
+      P.lo = plan_bound(d, sl);
+      P.hi = plan_bound(d, sh);
       // Both bounds in one bin: the result is the bin's list rows in [lo, hi] alone,
       // (1 ^ t_hi) & ~(1 ^ t_lo) = t_lo & ~t_hi, so no filter vector is read.
       if (sl.fv == sh.fv && sl.list >= 0 && sl.list == sh.list) {
-        bc.lo.base = bc.hi.base = kBaseOnes;
-        bc.lo.fv = bc.hi.fv = nullptr;
-        p->stats_narrow_cols++;
+        P.lo.base = P.hi.base = kBaseOnes;
+        P.lo.fv = P.hi.fv = nullptr;
+        st_narrow++;
       }
-      p->stats_bindex_cols++;
+      st_bindex++;
     }
   }
-  p->nq = nq;
-  p->specs.assign(queries, queries + nq);
-  // Work-item groups: consecutive queries while their columns fit the LDS slots of one tile
-  // and the configured group size.
-  const int slots = kBatchLdsWords / p->cfg.tile_words;
-  p->h_groups.clear();
+  // 2. groups: consecutive queries while their slots fit the LDS of one tile and the
+  //    configured group size; slots and lists flattened per group
+  const int slot_cap = kBatchLdsWords / p->cfg.tile_words;
+  const int64_t per = p->ntiles + 1;
+  std::vector<BatchGroup> groups;
+  std::vector<BatchSlot> slots;
+  std::vector<BatchList> lists;
+  std::vector<BatchQuery> qs(nq);
   for (int q = 0; q < nq;) {
     BatchGroup g{};
     g.q0 = q;
+    g.slot0 = (int)slots.size();
+    g.list0 = (int)lists.size();
     while (q < nq && g.nq < p->cfg.queries_per_group) {
-      const int need = p->h_queries[q].empty ? 0 : p->h_queries[q].ncols;
-      if (g.nslots + need > slots) break;
+      const int need = qempty[q] ? 0 : qcols[q];
+      if (g.nslots + need > slot_cap) break;
+      g.qslot[g.nq] = (uint8_t)g.nslots;
       for (int c = 0; c < need; c++) {
-        g.sq[g.nslots] = (uint8_t)g.nq;
-        g.sc[g.nslots] = (uint8_t)c;
+        const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
+        BatchSlot S{};
+        if (P.direct) {
+          S.col = p->columns[P.col];
+          S.dlo = P.dlo;
+          S.dspan = P.dspan;
+        } else {
+          S.fvhi = P.hi.fv;
+          S.fvlo = P.lo.fv;
+          S.hibase = P.hi.base;
+          S.lobase = P.lo.base;
+          const rts_bindex_dev *d = p->index[P.col]->dev;
+          for (const PlannedBound *B : {&P.lo, &P.hi}) {
+            if (B->list < 0) continue;
+            BatchList L{};
+            L.dir = p->dir_base[P.col] + (size_t)B->list * per;
+            L.bpos = d->bpos;
+            L.bval = d->bval;
+            L.thr = B->thr;
+            L.slot = (uint32_t)g.nslots;
+            lists.push_back(L);
+            g.nlists++;
+          }
+        }
+        slots.push_back(S);
         g.nslots++;
       }
+      qs[q].out = outs ? outs[q] : nullptr;
+      qs[q].empty = qempty[q];
       g.nq++;
       q++;
     }
-    p->h_groups.push_back(g);
+    g.qslot[g.nq] = (uint8_t)g.nslots;
+    groups.push_back(g);
   }
+  p->has_outs = outs != nullptr;
+  p->outs.assign(outs ? outs : (BITS *const *)nullptr, outs ? outs + nq : (BITS *const *)nullptr);
+  p->stats_bindex_cols = st_bindex;
+  p->stats_direct_cols = st_direct;
+  p->stats_narrow_cols = st_narrow;
+  p->nq = nq;
+  p->specs.assign(queries, queries + nq);
+  p->h_queries.swap(qs);
+  p->h_groups.swap(groups);
+  p->h_slots.swap(slots);
+  p->h_lists.swap(lists);
   p->ngroups = (int)p->h_groups.size();
   if (nq == 0) return hipSuccess;
   hipError_t e = hipMemcpyAsync(p->d_queries, p->h_queries.data(), (size_t)nq * sizeof(BatchQuery),
@@ -604,7 +732,15 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   if (e == hipSuccess)
     e = hipMemcpyAsync(p->d_groups, p->h_groups.data(), (size_t)p->ngroups * sizeof(BatchGroup),
                        hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);  // h_queries may change on the next call
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(p->d_ngroups, &p->ngroups, sizeof(int), hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && !p->h_slots.empty())
+    e = hipMemcpyAsync(p->d_slots, p->h_slots.data(), p->h_slots.size() * sizeof(BatchSlot),
+                       hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess && !p->h_lists.empty())
+    e = hipMemcpyAsync(p->d_lists, p->h_lists.data(), p->h_lists.size() * sizeof(BatchList),
+                       hipMemcpyHostToDevice, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the host vectors change on the next call
   return set_error(e, "rts_batch_plan_set_queries");
 }
 
@@ -671,6 +807,9 @@ void rts_batch_plan_free(rts_batch_plan *p) {
   free_dirs(p);
   (void)hipFree(p->d_queries);
   (void)hipFree(p->d_groups);
+  (void)hipFree(p->d_slots);
+  (void)hipFree(p->d_lists);
+  (void)hipFree(p->d_ngroups);
   (void)hipFree(p->d_partials);
   delete p;
 }
