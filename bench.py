@@ -350,7 +350,8 @@ def batch_queries(table, nq, seed, direct_share):
     return qs
 
 
-def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cache=None):
+def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cache=None,
+                config=None):
     """Config 4: nq mixed queries in one persistent hipGraph-captured launch, launch shape
     from the autotuner, against the plain one-query-per-launch paths (direct conjunction,
     and BinDex) on the same GPU."""
@@ -363,7 +364,10 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
     counts = R.DeviceBuffer(8 * nq)
     plan = R.BatchPlan(table.dcols, table.idx, n, nq, None, stream)
     plan.set_queries(qs, stream)
-    cands = [R.batch_config(t, g, b) for t in (1024, 2048) for g in (8, 16, 32) for b in (2, 4)]
+    if config:  # fixed launch shape (profiling runs)
+        cands = [R.batch_config(*config)]
+    else:
+        cands = [R.batch_config(t, g, b) for t in (1024, 2048) for g in (8, 16, 32) for b in (2, 4)]
     t0 = time.perf_counter()
     chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,
                                     stream=stream)
@@ -533,6 +537,9 @@ def main():
                     choices=["scan", "copycal", "conj", "batch", "serve"])
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
+    ap.add_argument("--batch-config", default=None,
+                    help="tile_words,queries_per_group,blocks_per_cu for the batch workload "
+                         "(skips the autotuner)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction and batched legs of the default run")
     args = ap.parse_args()
@@ -556,7 +563,8 @@ def main():
         if args.workload == "conj":
             res["conjunction"] = bench_conj(R, table)
         else:
-            res["batch"] = bench_batch(R, table, steps=max(1, min(args.steps, 20)))
+            cfg = tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config else None
+            res["batch"] = bench_batch(R, table, steps=max(1, min(args.steps, 20)), config=cfg)
     else:
         res = bench_scan(args, R)
         if not args.no_extra:

@@ -32,7 +32,6 @@ enum : uint32_t { kBaseZero = 0, kBaseOnes = 1, kBaseFv = 2 };
 
 constexpr int kBatchThreads = 512;  // 8 waves per workgroup, 2 workgroups per CU (LDS)
 constexpr int kBatchWaves = kBatchThreads / kWave;
-constexpr int kEntryUnroll = 8;     // refine entries per thread per round
 
 // One query-column of a group.  BinDex form: (fv_hi & ~fv_lo) ^ T, with T the toggles of
 // the slot's refine lists; bases kBaseZero / kBaseOnes need no filter vector.  Direct form
@@ -46,44 +45,48 @@ struct BatchSlot {
 };
 
 // One refine list of a group: the rows of a bin's B list (in row order, tile directory
-// `dir`) with value >= thr flip bits of slot `slot`.
+// `dir`) with value >= thr flip bits of slot `slot`.  Unused entries point at a zero
+// directory (no rows), so the kernel loads every entry without first reading the count.
 struct BatchList {
   const uint32_t *dir;
-  const uint32_t *bpos;
-  const CODE *bval;
+  const uint2 *bent;  // (row, value) entries of the column's B lists
   CODE thr;
   uint32_t slot;
 };
 
-// Output of one query: its own bitmap, or out + q * out_stride of the launch.
+// One query of a group: its own bitmap (or out + id * out_stride of the launch) and its
+// index `id` in the caller's order (bitmap row and count index).
 struct BatchQuery {
   BITS *out;
   int empty;  // some column's range is empty: all-zero result
-  int pad;
+  int id;
 };
 
-// A work item's query group: queries [q0, q0 + nq) with slots [slot0, slot0 + nslots) and
-// lists [list0, list0 + nlists) of the plan; query i owns slots [qslot[i], qslot[i + 1]).
+// A work item's query group, stored contiguously so that one round of loads brings in all
+// of it: queries query[0..nq), query i owning slots [qslot[i], qslot[i + 1]), and the refine
+// lists of those slots.
 struct BatchGroup {
   int q0, nq, nslots, nlists;
-  int slot0, list0;
-  uint8_t qslot[kBatchMaxGroup + 1];
+  uint8_t qslot[80];
+  BatchSlot slot[kBatchMaxSlots];
+  BatchList list[2 * kBatchMaxSlots];
+  BatchQuery query[kBatchMaxGroup];
 };
 
 // LDS of one workgroup: one toggle array per slot and the current item's descriptors.
 template <int TW>
 struct BatchLds {
   static constexpr int kSlots = kBatchLdsWords / TW;
+  static constexpr int kLists = 2 * kSlots;
   uint32_t tog[kSlots][TW];
-  const uint32_t *lpos[2 * kSlots];
-  const CODE *lval[2 * kSlots];
-  uint32_t lstart[2 * kSlots];
-  uint32_t lpref[2 * kSlots + 1];
-  CODE lthr[2 * kSlots];
-  uint32_t lslot[2 * kSlots];
+  const uint2 *lent[kLists];  // first entry of the list's tile slice
+  uint32_t llen[kLists];      // entries in the slice
+  CODE lthr[kLists];
+  uint32_t lslot[kLists];
   BatchSlot slot[kSlots];
   BITS *qout[kBatchMaxGroup];
   uint32_t qempty[kBatchMaxGroup];
+  uint32_t qid[kBatchMaxGroup];
   uint32_t qslot[kBatchMaxGroup + 1];
   uint32_t qcount[kBatchMaxGroup];
 };
@@ -165,27 +168,34 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
   return r;
 }
 
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
+// Result words are written once and not read again by the launch: non-temporal stores keep
+// them from evicting the filter-vector and list slices that the tile's other query groups
+// re-read from L2.
 template <int KP>
 __device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
   if constexpr (KP == 2) {
-    *reinterpret_cast<uint2 *>(o) = make_uint2(acc[0], acc[1]);
+    u32x2 v = {acc[0], acc[1]};
+    __builtin_nontemporal_store(v, reinterpret_cast<u32x2 *>(o));
   } else {
 #pragma unroll
-    for (int k = 0; k < KP; k += 4)
-      *reinterpret_cast<uint4 *>(o + k) = make_uint4(acc[k], acc[k + 1], acc[k + 2], acc[k + 3]);
+    for (int k = 0; k < KP; k += 4) {
+      u32x4 v = {acc[k], acc[k + 1], acc[k + 2], acc[k + 3]};
+      __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(o + k));
+    }
   }
 }
 
 template <int TW>
 __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
-    const BatchGroup *__restrict__ groups, const BatchSlot *__restrict__ slots,
-    const BatchList *__restrict__ lists, const BatchQuery *__restrict__ queries,
-    const int *__restrict__ ngroups_ptr, int64_t n, int64_t words, int64_t ntiles, BITS *out,
-    int64_t out_stride, uint32_t *partials) {
+    const BatchGroup *__restrict__ groups, const int *__restrict__ ngroups_ptr, int64_t n,
+    int64_t words, int64_t ntiles, BITS *out, int64_t out_stride, uint32_t *partials) {
   constexpr int kPer = TW / kBatchThreads;  // output words per thread
   static_assert(kPer == 2 || kPer == 4, "tile words per thread");
   using Lds = BatchLds<TW>;
-  constexpr int kSlots = Lds::kSlots;
+  constexpr int kSlots = Lds::kSlots, kLists = Lds::kLists;
+  static_assert(kLists <= kWave && kSlots <= kWave, "descriptor loads by one wave each");
   __shared__ Lds lds;
   const int tid = threadIdx.x;
   for (int i = tid; i < kSlots * TW; i += kBatchThreads) (&lds.tog[0][0])[i] = 0;
@@ -201,71 +211,104 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
   __syncthreads();
   for (int64_t it = j; it < items_x; it += nblk_x) {
     const int64_t t = xcd + kXcds * (it / ngroups);
-    const BatchGroup &g = groups[it % ngroups];
-    const int nslots = g.nslots, nlists = g.nlists, nq = g.nq, q0 = g.q0;
+    const BatchGroup *g = groups + it % ngroups;
     const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
 
-    // 1. descriptors of the item into LDS: lists (and their tile slices), slots, queries
-    if (tid < nlists) {
-      const BatchList L = lists[g.list0 + tid];
+    // 1. the item's descriptors into LDS: every load of this round is independent of the
+    //    others; the list's tile slice (two directory words) is the only second round
+    if (tid < kLists) {
+      const BatchList L = g->list[tid];
       const uint32_t st = L.dir[t];
-      lds.lstart[tid] = st;
-      lds.lpref[tid + 1] = L.dir[t + 1] - st;
-      lds.lpos[tid] = L.bpos;
-      lds.lval[tid] = L.bval;
+      lds.lent[tid] = L.bent + st;
+      lds.llen[tid] = L.dir[t + 1] - st;
       lds.lthr[tid] = L.thr;
       lds.lslot[tid] = L.slot;
-    } else if (tid >= kWave && tid < kWave + nslots) {
-      lds.slot[tid - kWave] = slots[g.slot0 + tid - kWave];
-    } else if (tid >= 2 * kWave && tid < 2 * kWave + nq) {
+    } else if (tid >= kWave && tid < kWave + kSlots) {
+      lds.slot[tid - kWave] = g->slot[tid - kWave];
+    } else if (tid >= 2 * kWave && tid < 2 * kWave + kBatchMaxGroup) {
       const int qi = tid - 2 * kWave;
-      const BatchQuery Q = queries[q0 + qi];
-      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)(q0 + qi) * out_stride;
+      const BatchQuery Q = g->query[qi];
+      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)Q.id * out_stride;
       lds.qempty[qi] = Q.empty;
+      lds.qid[qi] = (uint32_t)Q.id;
+    } else if (tid >= 4 * kWave && tid <= 4 * kWave + kBatchMaxGroup) {
+      lds.qslot[tid - 4 * kWave] = g->qslot[tid - 4 * kWave];
     }
-    if (tid >= 3 * kWave && tid <= 3 * kWave + nq) lds.qslot[tid - 3 * kWave] = g.qslot[tid - 3 * kWave];
-    __syncthreads();
-    if (tid < kWave) {  // inclusive scan of the list lengths (nlists <= 32)
-      uint32_t v = tid < nlists ? lds.lpref[tid + 1] : 0;
-#pragma unroll
-      for (int o = 1; o < kWave; o <<= 1) {
-        const uint32_t u = __shfl_up(v, o, kWave);
-        if (tid >= o) v += u;
-      }
-      if (tid < nlists) lds.lpref[tid + 1] = v;
-      if (tid == 0) lds.lpref[0] = 0;
-    }
+    const int nslots = g->nslots, nq = g->nq;
     __syncthreads();
 
-    // 2. refine toggles of every list, kEntryUnroll entries per thread with their loads
-    //    issued together; direct slots write their predicate words
-    const uint32_t total = lds.lpref[nlists];
-    for (uint32_t e0 = 0; e0 < total; e0 += kBatchThreads * kEntryUnroll) {
-      uint32_t pos[kEntryUnroll], val[kEntryUnroll], lid[kEntryUnroll];
+    // 2. filter-vector words of every BinDex slot (loads issued first), then the refine
+    //    toggles, several entries per thread with their loads issued together
+    const int64_t w = t * TW + lw;
+    // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time (their
+    //    pointers come from the group record through scalar loads), XORed into the toggles
+    //    as they arrive: every toggle update is an XOR, so the order does not matter and the
+    //    words need not stay in registers.  Refine entries: wave v owns lists v, v + 8, ...;
+    //    a list's slice base is uniform across the wave, each lane takes kListUnroll entries
+    //    of each of its lists per round, and the first round's loads go out together with
+    //    the first 8 slots' filter-vector loads.
+    constexpr int kListsPerWave = kLists / kBatchWaves;
+    constexpr int kListUnroll = 16 / kListsPerWave;
+    static_assert(kLists % kBatchWaves == 0, "lists per wave");
+    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    const uint2 *lbase[kListsPerWave];
+    uint32_t llen[kListsPerWave], lmax = 0;
 #pragma unroll
-      for (int u = 0; u < kEntryUnroll; u++) {
-        const uint32_t e = e0 + u * kBatchThreads + tid;
-        lid[u] = 0xFFFFFFFFu;
-        if (e < total) {
-          int lo = 0, hi = nlists - 1;  // last list with lpref[l] <= e
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (lds.lpref[mid] <= e)
-              lo = mid;
-            else
-              hi = mid - 1;
+    for (int i = 0; i < kListsPerWave; i++) {
+      const int L = wv + i * kBatchWaves;
+      // the slice base is the same in every lane: move it to scalar registers, one 32-bit
+      // half at a time (the builtin takes and returns 32-bit values)
+      const uint64_t a = (uint64_t)lds.lent[L];
+      const uint32_t a_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
+      const uint32_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
+      lbase[i] = (const uint2 *)(((uint64_t)a_hi << 32) | a_lo);
+      llen[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)lds.llen[L]);
+      lmax = llen[i] > lmax ? llen[i] : lmax;
+    }
+    for (uint32_t off = 0, first = 1; off < lmax || first; off += kWave * kListUnroll, first = 0) {
+      uint2 ent[kListsPerWave][kListUnroll];
+#pragma unroll
+      for (int i = 0; i < kListsPerWave; i++)
+#pragma unroll
+        for (int u = 0; u < kListUnroll; u++) {
+          const uint32_t e = off + u * kWave + lane;
+          ent[i][u] = e < llen[i] ? lbase[i][e] : make_uint2(0, 0);
+        }
+      if (first) {
+        constexpr int kFvBatch = 16 / kPer;  // slots whose words are in flight together
+#pragma unroll
+        for (int s0 = 0; s0 < kSlots; s0 += kFvBatch) {
+          Words<kPer> h[kFvBatch], l[kFvBatch];
+#pragma unroll
+          for (int u = 0; u < kFvBatch; u++) {
+            const BatchSlot &S = g->slot[s0 + u];
+            if (s0 + u < nslots && !S.col && w < words) {
+              h[u] = base_words<kPer>(S.fvhi, S.hibase, w, words);
+              l[u] = base_words<kPer>(S.fvlo, S.lobase, w, words);
+            }
           }
-          const uint32_t idx = lds.lstart[lo] + (e - lds.lpref[lo]);
-          pos[u] = lds.lpos[lo][idx];
-          val[u] = lds.lval[lo][idx];
-          lid[u] = (uint32_t)lo;
+#pragma unroll
+          for (int u = 0; u < kFvBatch; u++) {
+            const BatchSlot &S = g->slot[s0 + u];
+            if (s0 + u < nslots && !S.col && w < words) {
+#pragma unroll
+              for (int k = 0; k < kPer; k++)
+                atomicXor(&lds.tog[s0 + u][lw + k], h[u].v[k] & ~l[u].v[k]);
+            }
+          }
         }
       }
 #pragma unroll
-      for (int u = 0; u < kEntryUnroll; u++) {
-        if (lid[u] != 0xFFFFFFFFu && val[u] >= lds.lthr[lid[u]]) {
-          const uint32_t p = pos[u];
-          atomicXor(&lds.tog[lds.lslot[lid[u]]][(p - row0) >> 5], 0x80000000u >> (p & 31));
+      for (int i = 0; i < kListsPerWave; i++) {
+        const int L = wv + i * kBatchWaves;
+        const uint32_t thr = lds.lthr[L];
+        uint32_t *tg = lds.tog[lds.lslot[L]];
+#pragma unroll
+        for (int u = 0; u < kListUnroll; u++) {
+          const uint32_t e = off + u * kWave + lane;
+          // rows of the slice lie in [row0, row0 + TW * 32): the word index is < TW
+          if (e < llen[i] && ent[i][u].y >= thr)
+            atomicXor(&tg[(ent[i][u].x - row0) >> 5], 0x80000000u >> (ent[i][u].x & 31));
         }
       }
     }
@@ -273,30 +316,8 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
       if (lds.slot[s].col) batch_direct<TW>(lds.slot[s], t, n, lds.tog[s]);
     __syncthreads();
 
-    // 3a. each BinDex slot's words: tog ^= fv_hi & ~fv_lo, 4 slots' loads in flight.  Every
-    //     thread touches only its own words from here on.
-    const int64_t w = t * TW + lw;
-    for (int s0 = 0; s0 < nslots; s0 += 4) {
-      Words<kPer> h[4], l[4];
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int s = s0 + u;
-        if (s < nslots && !lds.slot[s].col && w < words) {
-          h[u] = base_words<kPer>(lds.slot[s].fvhi, lds.slot[s].hibase, w, words);
-          l[u] = base_words<kPer>(lds.slot[s].fvlo, lds.slot[s].lobase, w, words);
-        }
-      }
-#pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const int s = s0 + u;
-        if (s < nslots && !lds.slot[s].col && w < words) {
-#pragma unroll
-          for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] ^= h[u].v[k] & ~l[u].v[k];
-        }
-      }
-    }
-
-    // 3b. each query: AND of its slots, stored with its popcount
+    // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
+    //    its own words from here on.
     for (int qi = 0; qi < nq; qi++) {
       uint32_t acc[kPer];
       const uint32_t fill = lds.qempty[qi] ? 0u : ~0u;
@@ -331,7 +352,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
       for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] = 0;
     __syncthreads();
     if (tid < nq) {  // read and cleared by the same thread: the next item starts at 0
-      partials[(int64_t)(q0 + tid) * ntiles + t] = lds.qcount[tid];
+      partials[(int64_t)lds.qid[tid] * ntiles + t] = lds.qcount[tid];
       lds.qcount[tid] = 0;
     }
   }
@@ -399,17 +420,12 @@ struct rts_batch_plan {
   int nq = 0;
   rts_batch_config cfg{};
   int64_t ntiles = 0;
-  BatchQuery *d_queries = nullptr;
-  BatchGroup *d_groups = nullptr;
-  BatchSlot *d_slots = nullptr;  // max_queries * kBatchMaxCols
-  BatchList *d_lists = nullptr;  // max_queries * kBatchMaxCols * 2
-  int *d_ngroups = nullptr;      // device copy of ngroups
+  BatchGroup *d_groups = nullptr;  // max_queries records
+  int *d_ngroups = nullptr;        // device copy of ngroups
+  uint32_t *d_zero_dir = nullptr;  // ntiles + 1 zeros: the directory of an unused list
   int ngroups = 0;
   uint32_t *d_partials = nullptr;
-  std::vector<BatchQuery> h_queries;
   std::vector<BatchGroup> h_groups;
-  std::vector<BatchSlot> h_slots;
-  std::vector<BatchList> h_lists;
   std::vector<uint32_t *> dir_base;  // per column: bdir re-tiled for cfg.tile_words
   std::vector<int64_t> dir_tiles;
   hipGraph_t graph = nullptr;
@@ -428,7 +444,7 @@ namespace rts {
 
 // Tile directory of one column for tiles of `tile_rows` rows (the index's own directory is
 // for kTileRows): first B-list entry of bin k with row >= t * tile_rows.
-__global__ void k_retile_directory(const uint32_t *bpos, const uint32_t *boff, int64_t ntiles,
+__global__ void k_retile_directory(const uint2 *bent, const uint32_t *boff, int64_t ntiles,
                                    int64_t tile_rows, uint32_t *bdir) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = ntiles + 1;
@@ -438,7 +454,7 @@ __global__ void k_retile_directory(const uint32_t *bpos, const uint32_t *boff, i
   uint32_t lo = boff[k], hi = boff[k + 1];
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
-    if (bpos[mid] < row)
+    if (bent[mid].x < row)
       lo = mid + 1;
     else
       hi = mid;
@@ -455,6 +471,8 @@ static void free_dirs(rts_batch_plan *p) {
   for (uint32_t *d : p->dir_base) (void)hipFree(d);
   p->dir_base.clear();
   p->dir_tiles.clear();
+  (void)hipFree(p->d_zero_dir);
+  p->d_zero_dir = nullptr;
 }
 
 static hipError_t build_dirs(rts_batch_plan *p, hipStream_t s) {
@@ -463,6 +481,9 @@ static hipError_t build_dirs(rts_batch_plan *p, hipStream_t s) {
   const int64_t ntiles = (p->n + tile_rows - 1) / tile_rows;
   p->dir_base.assign(p->ncols, nullptr);
   p->dir_tiles.assign(p->ncols, ntiles);
+  hipError_t ze = hipMalloc((void **)&p->d_zero_dir, (ntiles + 1) * sizeof(uint32_t));
+  if (ze == hipSuccess) ze = hipMemsetAsync(p->d_zero_dir, 0, (ntiles + 1) * sizeof(uint32_t), s);
+  if (ze != hipSuccess) return ze;
   for (int c = 0; c < p->ncols; c++) {
     const BinDex *b = p->index[c];
     if (!b || !b->dev) continue;
@@ -471,7 +492,7 @@ static hipError_t build_dirs(rts_batch_plan *p, hipStream_t s) {
     hipError_t e = hipMalloc((void **)&p->dir_base[c], items * sizeof(uint32_t));
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_retile_directory, dim3((unsigned)((items + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, d->bpos, d->boff, ntiles, tile_rows, p->dir_base[c]);
+                       0, s, d->bent, d->boff, ntiles, tile_rows, p->dir_base[c]);
     e = hipGetLastError();
     if (e != hipSuccess) return e;
   }
@@ -506,14 +527,12 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
   grid = std::max<unsigned>(kXcds, grid / kXcds * kXcds);
   switch (p->cfg.tile_words) {
     case 2048:
-      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_slots,
-                         p->d_lists, p->d_queries, p->d_ngroups, p->n, p->words, p->ntiles, out, out_stride,
-                         p->d_partials);
+      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
+                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
       break;
     default:
-      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_slots,
-                         p->d_lists, p->d_queries, p->d_ngroups, p->n, p->words, p->ntiles, out, out_stride,
-                         p->d_partials);
+      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
+                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !counts) return e;
@@ -558,12 +577,7 @@ hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *column
     delete p;
     return set_error(hipErrorInvalidValue, "rts_batch_plan_create: config");
   }
-  hipError_t e = hipMalloc((void **)&p->d_queries, (size_t)max_queries * sizeof(BatchQuery));
-  if (e == hipSuccess) e = hipMalloc((void **)&p->d_groups, (size_t)max_queries * sizeof(BatchGroup));
-  if (e == hipSuccess)
-    e = hipMalloc((void **)&p->d_slots, (size_t)max_queries * kBatchMaxCols * sizeof(BatchSlot));
-  if (e == hipSuccess)
-    e = hipMalloc((void **)&p->d_lists, (size_t)max_queries * kBatchMaxCols * 2 * sizeof(BatchList));
+  hipError_t e = hipMalloc((void **)&p->d_groups, (size_t)max_queries * sizeof(BatchGroup));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_ngroups, sizeof(int));
   if (e == hipSuccess) e = build_dirs(p, stream);
   if (e == hipSuccess)
@@ -662,25 +676,22 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
     }
   }
   // 2. groups: consecutive queries while their slots fit the LDS of one tile and the
-  //    configured group size; slots and lists flattened per group
+  //    configured group size, each group one self-contained record
   const int slot_cap = kBatchLdsWords / p->cfg.tile_words;
   const int64_t per = p->ntiles + 1;
   std::vector<BatchGroup> groups;
-  std::vector<BatchSlot> slots;
-  std::vector<BatchList> lists;
-  std::vector<BatchQuery> qs(nq);
   for (int q = 0; q < nq;) {
-    BatchGroup g{};
+    BatchGroup g;
+    memset(&g, 0, sizeof(g));
+    for (BatchList &L : g.list) L.dir = p->d_zero_dir;
     g.q0 = q;
-    g.slot0 = (int)slots.size();
-    g.list0 = (int)lists.size();
     while (q < nq && g.nq < p->cfg.queries_per_group) {
       const int need = qempty[q] ? 0 : qcols[q];
       if (g.nslots + need > slot_cap) break;
       g.qslot[g.nq] = (uint8_t)g.nslots;
       for (int c = 0; c < need; c++) {
         const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
-        BatchSlot S{};
+        BatchSlot &S = g.slot[g.nslots];
         if (P.direct) {
           S.col = p->columns[P.col];
           S.dlo = P.dlo;
@@ -693,21 +704,19 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
           const rts_bindex_dev *d = p->index[P.col]->dev;
           for (const PlannedBound *B : {&P.lo, &P.hi}) {
             if (B->list < 0) continue;
-            BatchList L{};
+            BatchList &L = g.list[g.nlists++];
             L.dir = p->dir_base[P.col] + (size_t)B->list * per;
-            L.bpos = d->bpos;
-            L.bval = d->bval;
+            L.bent = d->bent;
             L.thr = B->thr;
             L.slot = (uint32_t)g.nslots;
-            lists.push_back(L);
-            g.nlists++;
           }
         }
-        slots.push_back(S);
         g.nslots++;
       }
-      qs[q].out = outs ? outs[q] : nullptr;
-      qs[q].empty = qempty[q];
+      BatchQuery &Q = g.query[g.nq];
+      Q.out = outs ? outs[q] : nullptr;
+      Q.empty = qempty[q];
+      Q.id = q;
       g.nq++;
       q++;
     }
@@ -721,25 +730,13 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   p->stats_narrow_cols = st_narrow;
   p->nq = nq;
   p->specs.assign(queries, queries + nq);
-  p->h_queries.swap(qs);
   p->h_groups.swap(groups);
-  p->h_slots.swap(slots);
-  p->h_lists.swap(lists);
   p->ngroups = (int)p->h_groups.size();
   if (nq == 0) return hipSuccess;
-  hipError_t e = hipMemcpyAsync(p->d_queries, p->h_queries.data(), (size_t)nq * sizeof(BatchQuery),
+  hipError_t e = hipMemcpyAsync(p->d_groups, p->h_groups.data(), (size_t)p->ngroups * sizeof(BatchGroup),
                                 hipMemcpyHostToDevice, stream);
   if (e == hipSuccess)
-    e = hipMemcpyAsync(p->d_groups, p->h_groups.data(), (size_t)p->ngroups * sizeof(BatchGroup),
-                       hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess)
     e = hipMemcpyAsync(p->d_ngroups, &p->ngroups, sizeof(int), hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess && !p->h_slots.empty())
-    e = hipMemcpyAsync(p->d_slots, p->h_slots.data(), p->h_slots.size() * sizeof(BatchSlot),
-                       hipMemcpyHostToDevice, stream);
-  if (e == hipSuccess && !p->h_lists.empty())
-    e = hipMemcpyAsync(p->d_lists, p->h_lists.data(), p->h_lists.size() * sizeof(BatchList),
-                       hipMemcpyHostToDevice, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);  // the host vectors change on the next call
   return set_error(e, "rts_batch_plan_set_queries");
 }
@@ -805,10 +802,7 @@ void rts_batch_plan_free(rts_batch_plan *p) {
   if (p->exec) (void)hipGraphExecDestroy(p->exec);
   if (p->graph) (void)hipGraphDestroy(p->graph);
   free_dirs(p);
-  (void)hipFree(p->d_queries);
   (void)hipFree(p->d_groups);
-  (void)hipFree(p->d_slots);
-  (void)hipFree(p->d_lists);
   (void)hipFree(p->d_ngroups);
   (void)hipFree(p->d_partials);
   delete p;

@@ -88,10 +88,13 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
   }
 }
 
-__global__ void k_gather_vals(const CODE *col, const uint32_t *pos, CODE *val, int64_t m) {
+// B-list entries: (row, value of the row).
+__global__ void k_gather_entries(const CODE *col, const uint32_t *pos, uint2 *ent, int64_t m) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m;
-       i += (int64_t)gridDim.x * blockDim.x)
-    val[i] = col[pos[i]];
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t p = pos[i];
+    ent[i] = make_uint2(p, col[p]);
+  }
 }
 
 // boff[k] = first sorted key >= k, for k in [0, kBins]
@@ -110,7 +113,7 @@ __global__ void k_bin_offsets(const uint8_t *skeys, int64_t n, uint32_t *boff) {
 }
 
 // bdir[k * (ntiles + 1) + t] = first entry of bin k with row >= t * kTileRows
-__global__ void k_tile_directory(const uint32_t *bpos, const uint32_t *boff, int64_t ntiles,
+__global__ void k_tile_directory(const uint2 *bent, const uint32_t *boff, int64_t ntiles,
                                  uint32_t *bdir) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = ntiles + 1;
@@ -121,7 +124,7 @@ __global__ void k_tile_directory(const uint32_t *bpos, const uint32_t *boff, int
   uint32_t lo = boff[k], hi = boff[k + 1];
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
-    if (bpos[mid] < row)
+    if (bent[mid].x < row)
       lo = mid + 1;
     else
       hi = mid;
@@ -160,8 +163,7 @@ static void free_dev(rts_bindex_dev *d) {
   (void)hipFree(d->sorted_val);
   (void)hipFree(d->sorted_pos);
   (void)hipFree(d->fvs);
-  (void)hipFree(d->bpos);
-  (void)hipFree(d->bval);
+  (void)hipFree(d->bent);
   (void)hipFree(d->boff);
   (void)hipFree(d->bdir);
   (void)hipFree(d->owned_column);
@@ -242,19 +244,17 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
   RTS_TRY(hipStreamSynchronize(s));
   nb = d->boff_host[kBins];
   d->nb = nb;
-  RTS_TRY(hipMalloc((void **)&d->bpos, std::max<uint32_t>(nb, 1) * sizeof(uint32_t)));
-  RTS_TRY(hipMalloc((void **)&d->bval, std::max<uint32_t>(nb, 1) * sizeof(CODE)));
+  RTS_TRY(hipMalloc((void **)&d->bent, std::max<uint32_t>(nb, 1) * sizeof(uint2)));
   if (nb) {
-    RTS_TRY(hipMemcpyAsync(d->bpos, bpos_in, nb * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
-    hipLaunchKernelGGL(k_gather_vals, dim3(blocks_for(nb)), dim3(kBlock), 0, s, col, d->bpos,
-                       d->bval, (int64_t)nb);
+    hipLaunchKernelGGL(k_gather_entries, dim3(blocks_for(nb)), dim3(kBlock), 0, s, col, bpos_in,
+                       d->bent, (int64_t)nb);
     RTS_TRY(hipGetLastError());
   }
   RTS_TRY(hipMalloc((void **)&d->bdir, (size_t)kBins * (d->ntiles + 1) * sizeof(uint32_t)));
   {
     const int64_t items = (int64_t)kBins * (d->ntiles + 1);
     hipLaunchKernelGGL(k_tile_directory, dim3((unsigned)((items + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, d->bpos, d->boff, d->ntiles, d->bdir);
+                       0, s, d->bent, d->boff, d->ntiles, d->bdir);
     RTS_TRY(hipGetLastError());
   }
 

@@ -6,8 +6,9 @@
 //   fvs                       K-1 filter vectors, fv[k] bit i = (x_i < start[k+1]),
 //                             fv_stride words apart (stride padded to 64 words)
 //   B lists                   for every bin k (value range [start[k], start[k+1])), the rows
-//                             with start[k] < x < start[k+1] in row order: bpos (row), bval
-//                             (value); bin k's entries are [boff[k], boff[k+1])
+//                             with start[k] < x < start[k+1] in row order, one 8-byte entry
+//                             per row: bent[e] = (row, value); bin k's entries are
+//                             [boff[k], boff[k+1])
 //   bdir                      tile directory: bin k's first entry with row >= t * kTileRows
 //                             is bdir[k * (ntiles + 1) + t]
 // A bound v of a query (the set L(v) = {x < v}) is then one filter vector plus the rows of
@@ -41,8 +42,7 @@ struct rts_bindex_dev {
   CODE *sorted_val = nullptr;
   int32_t *sorted_pos = nullptr;
   BITS *fvs = nullptr;
-  uint32_t *bpos = nullptr;
-  CODE *bval = nullptr;
+  uint2 *bent = nullptr;  // (row, value) per B-list entry
   uint32_t *boff = nullptr;  // device, kBins + 1
   uint32_t *bdir = nullptr;  // device, kBins * (ntiles + 1)
   CODE *owned_column = nullptr;

@@ -27,8 +27,7 @@ enum : uint32_t { kBaseZero = 0, kBaseOnes = 1, kBaseFv = 2 };
 struct BoundDev {
   const BITS *fv;       // kBaseFv
   const uint32_t *dir;  // tile directory row of the bin (nullptr: no refine)
-  const uint32_t *bpos;
-  const CODE *bval;
+  const uint2 *bent;    // (row, value) entries
   CODE thr;
   uint32_t base;
 };
@@ -44,8 +43,8 @@ __device__ __forceinline__ void mark_removed(const BoundDev &b, int64_t t, uint3
   const uint32_t e0 = b.dir[t], e1 = b.dir[t + 1];
   const uint32_t row0 = (uint32_t)(t * kTileRows);
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const uint32_t p = b.bpos[e];
-    if (b.bval[e] >= b.thr) atomicOr(&tog[(p - row0) >> 5], 0x80000000u >> (p & 31));
+    const uint2 pv = b.bent[e];
+    if (pv.y >= b.thr) atomicOr(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
   }
 }
 
@@ -128,8 +127,7 @@ static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   }
   if (s.list >= 0) {
     b.dir = d->bdir + (size_t)s.list * (d->ntiles + 1);
-    b.bpos = d->bpos;
-    b.bval = d->bval;
+    b.bent = d->bent;
     b.thr = s.thr;
   }
   return b;
