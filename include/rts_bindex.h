@@ -86,6 +86,14 @@ hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, h
 hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n);
 /* Release the device index (and the library-owned column copy, if any). */
 void rts_bindex_free(BinDex *bindex);
+/* Index persistence (no reference counterpart; the reference rebuilds at every start,
+ * bindex_cuda.cpp:1860-1890): write a built index to a binary snapshot file, and load one
+ * into device memory without sorting or rebuilding the filter vectors.  `dev_column` is the
+ * indexed column already on the device (kept as rawDataInGPU for the refine surface; may
+ * be NULL).  A snapshot whose header does not match (format, bin count, sizes) is refused
+ * with hipErrorInvalidValue.  Synchronous. */
+hipError_t rts_bindex_save(const BinDex *bindex, const char *path);
+hipError_t rts_bindex_load(BinDex *bindex, const char *path, const CODE *dev_column);
 
 /* Kernel (b) fused with kernel (c): out = AND over ncols indexed columns of
  * (lo[c] <= x_c <= hi[c]) evaluated from filter vectors and boundary-bin refine lists,

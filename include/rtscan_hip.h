@@ -96,6 +96,13 @@ hipError_t rts_memcpy_htod(void *dst, const void *src, size_t bytes);
 hipError_t rts_memcpy_dtoh(void *dst, const void *src, size_t bytes);
 hipError_t rts_memcpy_dtod(void *dst, const void *src, size_t bytes);
 hipError_t rts_memset(void *dst, int byte_value, size_t bytes);
+/* Result egress: page-locked host buffers (hipHostMalloc) and copies queued on a stream, so
+ * a bitmap's download overlaps the next query's kernels (the reference allocates pinned
+ * memory and copies synchronously, bindex_cuda.cpp:1933-1946). */
+hipError_t rts_host_alloc(void **ptr, size_t bytes);
+hipError_t rts_host_free(void *ptr);
+hipError_t rts_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, hipStream_t stream);
+hipError_t rts_memcpy_htod_async(void *dst, const void *src, size_t bytes, hipStream_t stream);
 hipError_t rts_device_synchronize(void);
 hipError_t rts_stream_create(hipStream_t *stream);
 hipError_t rts_stream_destroy(hipStream_t stream);

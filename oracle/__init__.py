@@ -87,6 +87,8 @@ def lib():
                                        ctypes.c_uint32]
         L.orc_bindex_scan.argtypes = [ctypes.POINTER(_OrcBindex), _u32p, ctypes.c_int,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
+        L.orc_bitmap_digest.argtypes = [_u32p, ctypes.c_int64]
+        L.orc_bitmap_digest.restype = ctypes.c_uint64
         L.orc_normal_encode.argtypes = [_u32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                         _u32p, _i32p, _u32p, _u32p, _u32p]
         L.orc_normal_encode.restype = ctypes.c_int64
@@ -366,3 +368,8 @@ def find_key_by_value(map_values, map_starts, val):
     return int(lib().orc_find_key_by_value(np.ascontiguousarray(map_values, np.uint32),
                                            np.ascontiguousarray(map_starts, np.uint32),
                                            len(map_values), val & 0xFFFFFFFF))
+
+
+def bitmap_digest(bitmap, n):
+    """The engine's 64-bit bitmap digest, restated (not a reference function)."""
+    return int(lib().orc_bitmap_digest(np.ascontiguousarray(bitmap, np.uint32), n))

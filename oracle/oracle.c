@@ -678,3 +678,25 @@ uint32_t orc_find_key_by_value(const uint32_t *map_values, const uint32_t *map_s
   }
   return 0;
 }
+
+/* ------------------------------------------------------------------ bitmap digest */
+
+static uint64_t mix64(uint64_t x) {
+  x ^= 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+
+uint64_t orc_bitmap_digest(const uint32_t *bitmap, int64_t n) {
+  if (n <= 0) return 0;
+  const int64_t words = (n + 31) / 32;
+  const uint32_t t = (uint32_t)(n & 31);
+  const uint32_t last = t == 0 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> t);
+  uint64_t d = 0;
+  for (int64_t i = 0; i < words; i++) {
+    const uint32_t w = i == words - 1 ? bitmap[i] & last : bitmap[i];
+    d += mix64(((uint64_t)i << 32) | w);
+  }
+  return d;
+}

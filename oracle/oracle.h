@@ -91,6 +91,11 @@ int orc_refine_area(const orc_bindex *const *bxs, const uint32_t *const *raws, i
 int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws, int ncols,
                        const uint32_t *predicate, int sel, int inverse, uint32_t *bitmap);
 
+/* Bitmap digest (the engine's rts_bitmap_digest): sum mod 2^64 over words i of
+ * mix64((i << 32) | word_i), bits past row n cleared, mix64(x) = splitmix64 finaliser of
+ * x ^ 0x9E3779B97F4A7C15.  Not a reference function: a checksum shared by both sides. */
+uint64_t orc_bitmap_digest(const uint32_t *bitmap, int64_t n);
+
 /* normalEncode, 7-argument form (remap.cpp:110-203), for one column: rank-order codes
  * code(row at sorted index i) = encode_min + i * interval (uint32 wrap), interval =
  * (int)((encode_max - encode_min) / n) in double; the encode map gives every distinct value

@@ -45,6 +45,10 @@ SIGNATURES = {
     "rts_memcpy_dtoh": (_err, [_vp, _vp, ctypes.c_size_t]),
     "rts_memcpy_dtod": (_err, [_vp, _vp, ctypes.c_size_t]),
     "rts_memset": (_err, [_vp, _int, ctypes.c_size_t]),
+    "rts_host_alloc": (_err, [ctypes.POINTER(_vp), ctypes.c_size_t]),
+    "rts_host_free": (_err, [_vp]),
+    "rts_memcpy_dtoh_async": (_err, [_vp, _vp, ctypes.c_size_t, _vp]),
+    "rts_memcpy_htod_async": (_err, [_vp, _vp, ctypes.c_size_t, _vp]),
     "rts_device_synchronize": (_err, []),
     "rts_stream_create": (_err, [ctypes.POINTER(_vp)]),
     "rts_stream_destroy": (_err, [_vp]),
@@ -79,6 +83,8 @@ SIGNATURES = {
     "rts_bindex_export_sorted": (_err, [_vp, _vp, _vp]),
     "rts_bindex_refine_entries": (ctypes.c_longlong, [_vp]),
     "rts_bindex_device_bytes": (ctypes.c_longlong, [_vp]),
+    "rts_bindex_save": (_err, [_vp, ctypes.c_char_p]),
+    "rts_bindex_load": (_err, [_vp, ctypes.c_char_p, _vp]),
     # query / data files
     "rts_query_file_parse": (_int, [ctypes.c_char_p, ctypes.c_size_t, _int, ctypes.POINTER(_vp),
                                     ctypes.POINTER(_int)]),
@@ -128,6 +134,10 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_int64), _vp]),
     "rts_encode_lookup": (_err, [_vp, _vp, ctypes.c_int64, _u32, _int, _u32, ctypes.POINTER(_u32)]),
     "rts_encode_key_of": (_u32, [_vp, _vp, ctypes.c_int64, _u32]),
+    # device-side checking
+    "rts_bitmap_compare": (_err, [_vp, _vp, ctypes.c_int64, _vp, _vp]),
+    "rts_bitmap_digest": (_err, [_vp, ctypes.c_int64, ctypes.POINTER(ctypes.c_ulonglong), _vp]),
+    "compare_bitmap": (None, [_vp, _vp, _int, _vp, _int]),
     # generators
     "rts_generate_column": (_int, [_int, _u64, _int, _u64, ctypes.c_double, ctypes.c_double,
                                    _vp, _int]),
@@ -329,6 +339,28 @@ def generate_column(dist, seed, column_id, n, p0=0.0, p1=0.0, threads=None):
     return out
 
 
+class _CompareResult(ctypes.Structure):
+    _fields_ = [("total_hit", ctypes.c_ulonglong), ("true_hit", ctypes.c_ulonglong),
+                ("first_mismatch", ctypes.c_longlong), ("truth_count", ctypes.c_ulonglong),
+                ("got_count", ctypes.c_ulonglong)]
+
+
+def bitmap_compare(truth, got, n, stream=None):
+    """compare_bitmap semantics (bindex.cpp:1959-1978) on device bitmaps: dict of
+    total_hit, true_hit, first_mismatch (-1: equal), truth_count, got_count."""
+    r = _CompareResult()
+    check(lib().rts_bitmap_compare(truth.ptr, got.ptr, n, ctypes.byref(r), _sh(stream)),
+          "rts_bitmap_compare")
+    return {f: getattr(r, f) for f, _ in _CompareResult._fields_}
+
+
+def bitmap_digest(bitmap, n, stream=None):
+    """64-bit position-sensitive digest of the first n rows of a device bitmap."""
+    d = ctypes.c_ulonglong()
+    check(lib().rts_bitmap_digest(bitmap.ptr, n, ctypes.byref(d), _sh(stream)), "rts_bitmap_digest")
+    return d.value
+
+
 def encode_column(col, n, encode_min, encode_max, codes=None, want_sorted=False, stream=None):
     """Rank encoding of a device column (normalEncode, remap.cpp:110-203).  Returns (codes,
     map_values, map_starts[, sorted_pos, sorted_codes]) as device buffers / numpy maps; codes
@@ -378,15 +410,24 @@ def encode_key_of(map_values, map_starts, code):
 class BinDexIndex:
     """A BinDex (include/rts_bindex.h) built on the GPU over a device column."""
 
-    def __init__(self, column, n, stream=None):
+    def __init__(self, column, n, stream=None, snapshot=None):
+        """Build over `column` (device, n rows), or load a snapshot written by save()."""
         from .bindex_struct import BinDex
 
         self.column = column  # keep the device column alive
         self.n = n
         self.s = BinDex()
-        check(lib().rts_bindex_build(ctypes.byref(self.s), column.ptr, n, _sh(stream)),
-              "rts_bindex_build")
+        if snapshot is not None:
+            check(lib().rts_bindex_load(ctypes.byref(self.s), os.fsencode(snapshot),
+                                        column.ptr if column is not None else None),
+                  "rts_bindex_load")
+        else:
+            check(lib().rts_bindex_build(ctypes.byref(self.s), column.ptr, n, _sh(stream)),
+                  "rts_bindex_build")
         self.built = True
+
+    def save(self, path):
+        check(lib().rts_bindex_save(ctypes.byref(self.s), os.fsencode(path)), "rts_bindex_save")
 
     @property
     def ptr(self):

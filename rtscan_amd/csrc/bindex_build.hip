@@ -9,6 +9,9 @@
 // row-ordered B lists, and a directory kernel.
 #include <hipcub/hipcub.hpp>
 
+#include <stdio.h>
+#include <string.h>
+
 #include <algorithm>
 #include <mutex>
 
@@ -170,6 +173,37 @@ static void free_dev(rts_bindex_dev *d) {
   delete d;
 }
 
+// The reference-shaped descriptor (bindex.h:126-138) over a complete device index: areas
+// and their blocks (bindex.cpp:633-652), start values, counts, filter-vector pointers.
+void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
+  d->area_store.resize(kBins);
+  int64_t b0 = 0;
+  for (int k = 0; k < kBins; k++) {
+    Area &a = d->area_store[k];
+    const int64_t len = d->area_len[k];
+    a.blockNum = (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
+    a.length = (int)len;
+    a.start = (POSTYPE)d->area_start[k];
+    a.blockFirst = d->block_first.data() + b0;
+    a.blockLast = d->block_last.data() + b0;
+    b0 += a.blockNum;
+  }
+  bx->length = (POSTYPE)d->n;
+  bx->data_min = d->starts[0];
+  for (int k = 0; k < kBins; k++) {
+    bx->areas[k] = &d->area_store[k];
+    bx->areaStartValues[k] = d->starts[k];
+    bx->area_counts[k] = (POSTYPE)(d->area_start[k] + d->area_len[k]);
+    bx->areasInGPU[k] = nullptr;
+  }
+  for (int k = 0; k < kBins - 1; k++) {
+    bx->filterVectors[k] = nullptr;
+    bx->filterVectorsInGPU[k] = d->fvs + (size_t)k * d->fv_stride;
+  }
+  bx->rawDataInGPU = const_cast<CODE *>(col);
+  bx->dev = d;
+}
+
 hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s) {
   hipError_t err = hipSuccess;
   rts_bindex_dev *d = nullptr;
@@ -290,36 +324,7 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
   RTS_TRY(hipMemcpyAsync(d->block_last.data(), blast_d, nblocks * sizeof(CODE), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipStreamSynchronize(s));
 
-  d->area_store.resize(kBins);
-  {
-    int64_t b0 = 0;
-    for (int k = 0; k < kBins; k++) {
-      Area &a = d->area_store[k];
-      const int64_t len = d->area_len[k];
-      a.blockNum = (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
-      a.length = (int)len;
-      a.start = (POSTYPE)d->area_start[k];
-      a.blockFirst = d->block_first.data() + b0;
-      a.blockLast = d->block_last.data() + b0;
-      b0 += a.blockNum;
-    }
-  }
-
-  // 6. fill the reference-shaped descriptor (bindex.h:126-138)
-  bx->length = (POSTYPE)n;
-  bx->data_min = d->starts[0];
-  for (int k = 0; k < kBins; k++) {
-    bx->areas[k] = &d->area_store[k];
-    bx->areaStartValues[k] = d->starts[k];
-    bx->area_counts[k] = (POSTYPE)(d->area_start[k] + d->area_len[k]);
-    bx->areasInGPU[k] = nullptr;
-  }
-  for (int k = 0; k < kBins - 1; k++) {
-    bx->filterVectors[k] = nullptr;
-    bx->filterVectorsInGPU[k] = d->fvs + (size_t)k * d->fv_stride;
-  }
-  bx->rawDataInGPU = const_cast<CODE *>(col);
-  bx->dev = d;
+  describe(bx, d, col);
   d = nullptr;
 
 fail:
@@ -345,6 +350,150 @@ void bindex_free(BinDex *bx) {
   if (!bx) return;
   free_dev(bx->dev);
   memset(bx, 0, sizeof(*bx));
+}
+
+}  // namespace rts
+
+// ----------------------------------------------------------------- snapshot (persistence)
+//
+// File layout (host byte order, little-endian on MI355X hosts):
+//   "RTSBDX" 0 1 | u32 version, u32 bins | i64 n, words, fv_stride, ntiles, nb, nblocks |
+//   u32 data_max | starts[bins] u32 | area_start[bins] i64 | area_len[bins] i64 |
+//   boff[bins + 1] u32 | block_first[nblocks] u32 | block_last[nblocks] u32 |
+//   sorted_val[n] u32 | sorted_pos[n] i32 | fvs[(bins - 1) * fv_stride] u32 | bent[nb] (u32, u32) |
+//   bdir[bins * (ntiles + 1)] u32
+// Device arrays stream through one page-locked staging buffer.
+
+namespace rts {
+namespace {
+
+constexpr char kSnapMagic[8] = {'R', 'T', 'S', 'B', 'D', 'X', 0, 1};
+constexpr uint32_t kSnapVersion = 1;
+constexpr size_t kStage = 64u << 20;
+
+struct SnapHeader {
+  char magic[8];
+  uint32_t version, bins;
+  int64_t n, words, fv_stride, ntiles, nb, nblocks;
+  uint32_t data_max, pad;
+};
+
+bool put(FILE *f, const void *p, size_t bytes) { return fwrite(p, 1, bytes, f) == bytes; }
+bool get(FILE *f, void *p, size_t bytes) { return fread(p, 1, bytes, f) == bytes; }
+
+hipError_t put_device(FILE *f, const void *dev, size_t bytes, void *stage) {
+  for (size_t off = 0; off < bytes; off += kStage) {
+    const size_t len = std::min(kStage, bytes - off);
+    hipError_t e = hipMemcpy(stage, (const char *)dev + off, len, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) return e;
+    if (!put(f, stage, len)) return hipErrorUnknown;
+  }
+  return hipSuccess;
+}
+
+hipError_t get_device(FILE *f, void *dev, size_t bytes, void *stage) {
+  for (size_t off = 0; off < bytes; off += kStage) {
+    const size_t len = std::min(kStage, bytes - off);
+    if (!get(f, stage, len)) return hipErrorInvalidValue;
+    hipError_t e = hipMemcpy((char *)dev + off, stage, len, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
+}  // namespace
+
+hipError_t bindex_save(const BinDex *bx, const char *path) {
+  if (!bx || !bx->dev || !path) return hipErrorInvalidValue;
+  const rts_bindex_dev *d = bx->dev;
+  FILE *f = fopen(path, "wb");
+  if (!f) return hipErrorInvalidValue;
+  void *stage = nullptr;
+  hipError_t e = hipHostMalloc(&stage, kStage, hipHostMallocDefault);
+  SnapHeader h{};
+  memcpy(h.magic, kSnapMagic, 8);
+  h.version = kSnapVersion;
+  h.bins = kBins;
+  h.n = d->n;
+  h.words = d->words;
+  h.fv_stride = d->fv_stride;
+  h.ntiles = d->ntiles;
+  h.nb = d->nb;
+  h.nblocks = (int64_t)d->block_first.size();
+  h.data_max = bx->data_max;
+  const size_t dir = (size_t)kBins * (d->ntiles + 1);
+  bool ok = e == hipSuccess && put(f, &h, sizeof(h)) && put(f, d->starts, sizeof(d->starts)) &&
+            put(f, d->area_start, sizeof(d->area_start)) && put(f, d->area_len, sizeof(d->area_len)) &&
+            put(f, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t)) &&
+            put(f, d->block_first.data(), h.nblocks * sizeof(CODE)) &&
+            put(f, d->block_last.data(), h.nblocks * sizeof(CODE));
+  if (ok) e = put_device(f, d->sorted_val, d->n * sizeof(CODE), stage);
+  if (ok && e == hipSuccess) e = put_device(f, d->sorted_pos, d->n * sizeof(int32_t), stage);
+  if (ok && e == hipSuccess)
+    e = put_device(f, d->fvs, (size_t)(kBins - 1) * d->fv_stride * sizeof(BITS), stage);
+  if (ok && e == hipSuccess) e = put_device(f, d->bent, d->nb * sizeof(uint2), stage);
+  if (ok && e == hipSuccess) e = put_device(f, d->bdir, dir * sizeof(uint32_t), stage);
+  if (stage) (void)hipHostFree(stage);
+  if (fclose(f) != 0) ok = false;
+  if (e != hipSuccess) return e;
+  return ok ? hipSuccess : hipErrorUnknown;
+}
+
+hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
+  if (!bx || !path) return hipErrorInvalidValue;
+  FILE *f = fopen(path, "rb");
+  if (!f) return hipErrorInvalidValue;
+  SnapHeader h{};
+  rts_bindex_dev *d = nullptr;
+  void *stage = nullptr;
+  hipError_t e = hipErrorInvalidValue;
+  size_t dir = 0;
+  if (!get(f, &h, sizeof(h)) || memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion ||
+      h.bins != (uint32_t)kBins || h.n < kBins || h.n > 0x7FFFFFFFll || h.words != (h.n + 31) / 32 ||
+      h.fv_stride < h.words || h.ntiles != (h.n + kTileRows - 1) / kTileRows || h.nb < 0 || h.nb > h.n ||
+      h.nblocks < kBins || h.nblocks > h.n)
+    goto done;
+  d = new rts_bindex_dev();
+  d->n = h.n;
+  d->words = h.words;
+  d->fv_stride = h.fv_stride;
+  d->ntiles = h.ntiles;
+  d->nb = h.nb;
+  d->boff_host.resize(kBins + 1);
+  d->block_first.resize(h.nblocks);
+  d->block_last.resize(h.nblocks);
+  if (!get(f, d->starts, sizeof(d->starts)) || !get(f, d->area_start, sizeof(d->area_start)) ||
+      !get(f, d->area_len, sizeof(d->area_len)) || !get(f, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t)) ||
+      !get(f, d->block_first.data(), h.nblocks * sizeof(CODE)) ||
+      !get(f, d->block_last.data(), h.nblocks * sizeof(CODE)))
+    goto done;
+  if (d->boff_host[kBins] != (uint32_t)h.nb) goto done;
+  dir = (size_t)kBins * (h.ntiles + 1);
+  e = hipHostMalloc(&stage, kStage, hipHostMallocDefault);
+  if (e == hipSuccess) e = hipMalloc((void **)&d->sorted_val, h.n * sizeof(CODE));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->sorted_pos, h.n * sizeof(int32_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->fvs, (size_t)(kBins - 1) * h.fv_stride * sizeof(BITS));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->bent, std::max<int64_t>(h.nb, 1) * sizeof(uint2));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->boff, (kBins + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->bdir, dir * sizeof(uint32_t));
+  if (e == hipSuccess) e = get_device(f, d->sorted_val, h.n * sizeof(CODE), stage);
+  if (e == hipSuccess) e = get_device(f, d->sorted_pos, h.n * sizeof(int32_t), stage);
+  if (e == hipSuccess) e = get_device(f, d->fvs, (size_t)(kBins - 1) * h.fv_stride * sizeof(BITS), stage);
+  if (e == hipSuccess) e = get_device(f, d->bent, h.nb * sizeof(uint2), stage);
+  if (e == hipSuccess) e = get_device(f, d->bdir, dir * sizeof(uint32_t), stage);
+  if (e == hipSuccess)
+    e = hipMemcpy(d->boff, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    memset(bx, 0, sizeof(*bx));
+    bx->data_max = h.data_max;
+    describe(bx, d, col);
+    d = nullptr;
+  }
+done:
+  if (stage) (void)hipHostFree(stage);
+  if (d) free_dev(d);
+  fclose(f);
+  return e;
 }
 
 }  // namespace rts

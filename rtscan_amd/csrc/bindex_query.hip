@@ -19,6 +19,8 @@ namespace rts {
 
 hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s);
 void bindex_free(BinDex *bx);
+hipError_t bindex_save(const BinDex *bx, const char *path);
+hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col);
 hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
                  hipStream_t s);
 
@@ -383,6 +385,14 @@ hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, h
   hipError_t e = bindex_build(bindex, dev_column, n, nullptr, stream);
   if (e == hipSuccess) e = hipStreamSynchronize(stream);
   return set_error(e, "rts_bindex_build");
+}
+
+hipError_t rts_bindex_save(const BinDex *bindex, const char *path) {
+  return set_error(bindex_save(bindex, path), "rts_bindex_save");
+}
+
+hipError_t rts_bindex_load(BinDex *bindex, const char *path, const CODE *dev_column) {
+  return set_error(bindex_load(bindex, path, dev_column), "rts_bindex_load");
 }
 
 hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n) {

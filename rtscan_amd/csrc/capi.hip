@@ -281,6 +281,16 @@ hipError_t rts_memcpy_htod(void *dst, const void *src, size_t bytes) {
 hipError_t rts_memcpy_dtoh(void *dst, const void *src, size_t bytes) {
   return set_error(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost), "rts_memcpy_dtoh");
 }
+hipError_t rts_host_alloc(void **ptr, size_t bytes) {
+  return set_error(hipHostMalloc(ptr, bytes, hipHostMallocDefault), "rts_host_alloc");
+}
+hipError_t rts_host_free(void *ptr) { return set_error(hipHostFree(ptr), "rts_host_free"); }
+hipError_t rts_memcpy_dtoh_async(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+  return set_error(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream), "rts_memcpy_dtoh_async");
+}
+hipError_t rts_memcpy_htod_async(void *dst, const void *src, size_t bytes, hipStream_t stream) {
+  return set_error(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "rts_memcpy_htod_async");
+}
 hipError_t rts_memcpy_dtod(void *dst, const void *src, size_t bytes) {
   return set_error(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "rts_memcpy_dtod");
 }

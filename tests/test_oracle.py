@@ -376,3 +376,32 @@ def test_refine_eq_area_covers_the_value_range():
         bm = np.zeros(ob.words, np.uint32)
         assert O.refine_area([ob], [x], [b, a], 0, False, bm, eq=True)
         assert not bm.any()
+
+
+def test_bitmap_digest_properties():
+    """The engine's bitmap checksum (not a reference function): it ignores bits past row n,
+    sees a single flipped bit and a swap of two different words, and is a sum of per-word
+    terms (the digest of a bitmap is the sum of the digests of its words kept alone, the
+    other words zeroed, minus the zero terms)."""
+    rng = np.random.default_rng(12)
+    n = 32 * 40 + 13
+    bm = rng.integers(0, 2 ** 32, (n + 31) // 32, dtype=np.uint64).astype(np.uint32)
+    d = O.bitmap_digest(bm, n)
+    tail = bm.copy()
+    tail[-1] ^= np.uint32(1)  # a bit past row n
+    assert O.bitmap_digest(tail, n) == d
+    for row in (0, 31, 32, 640, n - 1):
+        f = bm.copy()
+        f[row // 32] ^= np.uint32(1 << (31 - row % 32))
+        assert O.bitmap_digest(f, n) != d
+    sw = bm.copy()
+    sw[[3, 7]] = sw[[7, 3]]
+    assert O.bitmap_digest(sw, n) != d
+    zero = np.zeros_like(bm)
+    dz = O.bitmap_digest(zero, n)
+    parts = 0
+    for i in range(bm.size):
+        one = zero.copy()
+        one[i] = bm[i]
+        parts += O.bitmap_digest(one, n) - dz
+    assert (parts + dz) % 2 ** 64 == d
