@@ -367,7 +367,8 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
     if config:  # fixed launch shape (profiling runs)
         cands = [R.batch_config(*config)]
     else:
-        cands = [R.batch_config(t, g, b) for t in (1024, 2048) for g in (8, 16, 32) for b in (2, 4)]
+        cands = [R.batch_config(t, g, b) for t in (512, 1024, 2048) for g in (8, 16, 32, 64)
+                 for b in (2,)]
     t0 = time.perf_counter()
     chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,
                                     stream=stream)

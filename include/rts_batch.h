@@ -35,8 +35,8 @@ typedef struct rts_query_spec {
   int mode[RTS_BATCH_MAX_COLS];   /* enum rts_eval_mode */
 } rts_query_spec;
 
-/* Launch shape: tile_words output words per work item (1024 or 2048, i.e. 32768 or 65536
- * rows), queries_per_group queries per work item (1..64), blocks_per_cu persistent
+/* Launch shape: tile_words output words per work item (512, 1024 or 2048, i.e. 16384,
+ * 32768 or 65536 rows; a work item holds 16384 / tile_words query-columns), queries_per_group queries per work item (1..64), blocks_per_cu persistent
  * workgroups per compute unit (1..16). */
 typedef struct rts_batch_config {
   int tile_words;

@@ -9,7 +9,7 @@ namespace rts {
 constexpr int kBatchMaxCols = RTS_BATCH_MAX_COLS;  // columns per query
 constexpr int kBatchMaxGroup = 64;                 // queries per work item, upper bound
 constexpr int kBatchLdsWords = 16384;              // toggle words per workgroup (64 KB)
-constexpr int kBatchMaxSlots = 16;                 // column slots per work item (1024-word tiles)
+constexpr int kBatchMaxSlots = 32;                 // column slots per work item (512-word tiles)
 constexpr int kXcds = 8;                           // MI355X: 8 XCDs, workgroup b -> XCD b % 8
 
 }  // namespace rts

@@ -147,7 +147,9 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
     return r;
   }
   if (w + KP <= words) {
-    if constexpr (KP == 2) {
+    if constexpr (KP == 1) {
+      r.v[0] = fv[w];
+    } else if constexpr (KP == 2) {
       const uint2 x = *reinterpret_cast<const uint2 *>(fv + w);
       r.v[0] = x.x;
       r.v[1] = x.y;
@@ -175,7 +177,9 @@ typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 // re-read from L2.
 template <int KP>
 __device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
-  if constexpr (KP == 2) {
+  if constexpr (KP == 1) {
+    __builtin_nontemporal_store(acc[0], o);
+  } else if constexpr (KP == 2) {
     u32x2 v = {acc[0], acc[1]};
     __builtin_nontemporal_store(v, reinterpret_cast<u32x2 *>(o));
   } else {
@@ -192,7 +196,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
     const BatchGroup *__restrict__ groups, const int *__restrict__ ngroups_ptr, int64_t n,
     int64_t words, int64_t ntiles, BITS *out, int64_t out_stride, uint32_t *partials) {
   constexpr int kPer = TW / kBatchThreads;  // output words per thread
-  static_assert(kPer == 2 || kPer == 4, "tile words per thread");
+  static_assert(kPer == 1 || kPer == 2 || kPer == 4, "tile words per thread");
   using Lds = BatchLds<TW>;
   constexpr int kSlots = Lds::kSlots, kLists = Lds::kLists;
   static_assert(kLists <= kWave && kSlots <= kWave, "descriptor loads by one wave each");
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
           ent[i][u] = e < llen[i] ? lbase[i][e] : make_uint2(0, 0);
         }
       if (first) {
-        constexpr int kFvBatch = 16 / kPer;  // slots whose words are in flight together
+        constexpr int kFvBatch = kPer == 1 ? 8 : 16 / kPer;  // slots whose words are in flight together
 #pragma unroll
         for (int s0 = 0; s0 < kSlots; s0 += kFvBatch) {
           Words<kPer> h[kFvBatch], l[kFvBatch];
@@ -463,7 +467,7 @@ __global__ void k_retile_directory(const uint2 *bent, const uint32_t *boff, int6
 }
 
 static bool valid_config(const rts_batch_config &c) {
-  return (c.tile_words == 1024 || c.tile_words == 2048) && c.queries_per_group >= 1 &&
+  return (c.tile_words == 512 || c.tile_words == 1024 || c.tile_words == 2048) && c.queries_per_group >= 1 &&
          c.queries_per_group <= kBatchMaxGroup && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16;
 }
 
@@ -522,7 +526,9 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
   const int cus = std::max(1, device_cu_count());
   // a persistent grid larger than what is resident would leave its statically assigned
   // work items to a second round: clamp to the occupancy limit
-  const int resident = p->cfg.tile_words == 2048 ? batch_resident_per_cu<2048>() : batch_resident_per_cu<1024>();
+  const int resident = p->cfg.tile_words == 2048   ? batch_resident_per_cu<2048>()
+                       : p->cfg.tile_words == 1024 ? batch_resident_per_cu<1024>()
+                                                   : batch_resident_per_cu<512>();
   unsigned grid = (unsigned)(cus * std::min(p->cfg.blocks_per_cu, resident));
   grid = std::max<unsigned>(kXcds, grid / kXcds * kXcds);
   switch (p->cfg.tile_words) {
@@ -530,8 +536,12 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
       hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
                          p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
       break;
-    default:
+    case 1024:
       hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
+                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      break;
+    default:
+      hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
                          p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
   }
   hipError_t e = hipGetLastError();

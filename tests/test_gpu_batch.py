@@ -56,7 +56,8 @@ def check_results(eng, cols, queries, out, counts, stride):
 
 
 @pytest.mark.parametrize("tile_words,qgroup,direct_share", [
-    (1024, 16, 0.0), (1024, 4, 0.3), (2048, 8, 0.0), (2048, 64, 1.0), (1024, 1, 0.5)])
+    (1024, 16, 0.0), (1024, 4, 0.3), (2048, 8, 0.0), (2048, 64, 1.0), (1024, 1, 0.5),
+    (512, 32, 0.0), (512, 64, 0.4), (512, 3, 1.0)])
 def test_batch_matches_closed_form(eng, tile_words, qgroup, direct_share):
     n = (1 << 20) + 12345  # partial last tile, partial last word
     cols, dcols, idx, sc = make_table(eng, n)
@@ -126,7 +127,7 @@ def test_batch_autotune_and_cache(eng, tmp_path):
     stride = n // 32
     out = eng.DeviceBuffer(4 * stride * 48)
     counts = eng.DeviceBuffer(8 * 48)
-    cands = [eng.batch_config(t, g, b) for t in (1024, 2048) for g in (4, 16) for b in (2, 4)]
+    cands = [eng.batch_config(t, g, b) for t in (512, 1024, 2048) for g in (4, 16) for b in (2, 4)]
     cache = str(tmp_path / "tune.txt")
     chosen, ms = plan.autotune(out, stride, cands, reps=2, cache_path=cache)
     assert chosen.astuple() in [c.astuple() for c in cands] and ms > 0
