@@ -367,8 +367,8 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
     if config:  # fixed launch shape (profiling runs)
         cands = [R.batch_config(*config)]
     else:
-        cands = [R.batch_config(t, g, b) for t in (512, 1024, 2048) for g in (8, 16, 32, 64)
-                 for b in (2,)]
+        cands = [R.batch_config(t, g, 2, o) for t in (512, 1024, 2048) for g in (8, 16, 32, 64)
+                 for o in (0, 1)]
     t0 = time.perf_counter()
     chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,
                                     stream=stream)
@@ -539,8 +539,8 @@ def main():
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
-                    help="tile_words,queries_per_group,blocks_per_cu for the batch workload "
-                         "(skips the autotuner)")
+                    help="tile_words,queries_per_group,blocks_per_cu[,order] for the batch "
+                         "workload (skips the autotuner)")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction and batched legs of the default run")
     args = ap.parse_args()

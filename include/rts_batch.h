@@ -36,12 +36,16 @@ typedef struct rts_query_spec {
 } rts_query_spec;
 
 /* Launch shape: tile_words output words per work item (512, 1024 or 2048, i.e. 16384,
- * 32768 or 65536 rows; a work item holds 16384 / tile_words query-columns), queries_per_group queries per work item (1..64), blocks_per_cu persistent
- * workgroups per compute unit (1..16). */
+ * 32768 or 65536 rows; a work item holds 16384 / tile_words query-columns),
+ * queries_per_group queries per work item (1..64), blocks_per_cu persistent workgroups per
+ * compute unit (1..16), order 0: group the queries in the caller's order, 1: in an order
+ * that puts queries sharing filter vectors next to each other (results still land at the
+ * caller's query index). */
 typedef struct rts_batch_config {
   int tile_words;
   int queries_per_group;
   int blocks_per_cu;
+  int order;
 } rts_batch_config;
 
 typedef struct rts_batch_plan rts_batch_plan;

@@ -545,13 +545,13 @@ class QuerySpec(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("tile_words", ctypes.c_int), ("queries_per_group", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("order", ctypes.c_int)]
 
     def astuple(self):
-        return (self.tile_words, self.queries_per_group, self.blocks_per_cu)
+        return (self.tile_words, self.queries_per_group, self.blocks_per_cu, self.order)
 
 
-def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None):
+def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, order=None):
     c = BatchConfig()
     lib().rts_batch_default_config(ctypes.byref(c))
     if tile_words is not None:
@@ -560,6 +560,8 @@ def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None):
         c.queries_per_group = queries_per_group
     if blocks_per_cu is not None:
         c.blocks_per_cu = blocks_per_cu
+    if order is not None:
+        c.order = order
     return c
 
 

@@ -21,6 +21,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <vector>
 
 #include "batch.h"
@@ -354,9 +355,12 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
     for (int s = 0; s < nslots; s++)
 #pragma unroll
       for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] = 0;
+    // the next item's descriptors overwrite lds.qid as soon as its loads return: read it
+    // before the barrier
+    const uint32_t qid = tid < nq ? lds.qid[tid] : 0;
     __syncthreads();
     if (tid < nq) {  // read and cleared by the same thread: the next item starts at 0
-      partials[(int64_t)lds.qid[tid] * ntiles + t] = lds.qcount[tid];
+      partials[(int64_t)qid * ntiles + t] = lds.qcount[tid];
       lds.qcount[tid] = 0;
     }
   }
@@ -383,6 +387,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_counts(const uint32_t *partial
 // Planned form of one query-column before grouping.
 struct PlannedBound {
   uint32_t base = kBaseZero;
+  int fvk = -1;  // filter-vector index for kBaseFv
   const BITS *fv = nullptr;
   int list = -1;  // bin of the refine list, -1: none
   CODE thr = 0;
@@ -403,6 +408,7 @@ static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
     b.base = kBaseOnes;
   } else {
     b.base = kBaseFv;
+    b.fvk = s.fv;
     b.fv = d->fvs + (size_t)s.fv * d->fv_stride;
   }
   b.list = s.list;
@@ -467,7 +473,8 @@ __global__ void k_retile_directory(const uint2 *bent, const uint32_t *boff, int6
 }
 
 static bool valid_config(const rts_batch_config &c) {
-  return (c.tile_words == 512 || c.tile_words == 1024 || c.tile_words == 2048) && c.queries_per_group >= 1 &&
+  return (c.tile_words == 512 || c.tile_words == 1024 || c.tile_words == 2048) && (c.order == 0 || c.order == 1) &&
+         c.queries_per_group >= 1 &&
          c.queries_per_group <= kBatchMaxGroup && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16;
 }
 
@@ -559,6 +566,7 @@ void rts_batch_default_config(rts_batch_config *cfg) {
   cfg->tile_words = 1024;
   cfg->queries_per_group = 16;
   cfg->blocks_per_cu = 4;
+  cfg->order = 1;
 }
 
 hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *columns,
@@ -605,6 +613,7 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *
                                      hipStream_t stream) {
   if (!p || !cfg || !valid_config(*cfg)) return set_error(hipErrorInvalidValue, "rts_batch_plan_set_config");
   const bool retile = cfg->tile_words != p->cfg.tile_words;
+  const bool reorder = cfg->order != p->cfg.order;
   p->cfg = *cfg;
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -614,9 +623,9 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *
     (void)hipGraphDestroy(p->graph);
     p->graph = nullptr;
   }
-  if (!retile) return hipSuccess;
-  hipError_t e = build_dirs(p, stream);
-  if (e == hipSuccess) {
+  if (!retile && !reorder) return hipSuccess;
+  hipError_t e = retile ? build_dirs(p, stream) : hipSuccess;
+  if (e == hipSuccess && retile) {
     (void)hipFree(p->d_partials);
     p->d_partials = nullptr;
     e = hipMalloc((void **)&p->d_partials, (size_t)p->max_queries * p->ntiles * sizeof(uint32_t));
@@ -685,17 +694,38 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       st_bindex++;
     }
   }
-  // 2. groups: consecutive queries while their slots fit the LDS of one tile and the
-  //    configured group size, each group one self-contained record
+  // 2. the order queries are grouped in: the caller's, or sorted by the (column, lower and
+  //    upper filter vector) of their BinDex columns taken in column order, so that the
+  //    groups a tile's workgroups run together read few distinct filter vectors
+  std::vector<int> order(nq);
+  for (int q = 0; q < nq; q++) order[q] = q;
+  if (p->cfg.order == 1) {
+    std::vector<std::vector<int>> key(nq);
+    for (int q = 0; q < nq; q++) {
+      std::vector<std::array<int, 3>> cols;
+      for (int c = 0; c < qcols[q]; c++) {
+        const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
+        cols.push_back(P.direct ? std::array<int, 3>{P.col, kBins, kBins}
+                                : std::array<int, 3>{P.col, P.lo.fvk, P.hi.fvk});
+      }
+      std::sort(cols.begin(), cols.end());
+      for (const auto &a : cols) key[q].insert(key[q].end(), a.begin(), a.end());
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  // 3. groups: consecutive queries (in that order) while their slots fit the LDS of one
+  //    tile and the configured group size, each group one self-contained record
   const int slot_cap = kBatchLdsWords / p->cfg.tile_words;
   const int64_t per = p->ntiles + 1;
   std::vector<BatchGroup> groups;
-  for (int q = 0; q < nq;) {
+  for (int oi = 0; oi < nq;) {
+    int q = order[oi];
     BatchGroup g;
     memset(&g, 0, sizeof(g));
     for (BatchList &L : g.list) L.dir = p->d_zero_dir;
-    g.q0 = q;
-    while (q < nq && g.nq < p->cfg.queries_per_group) {
+    g.q0 = oi;
+    while (oi < nq && g.nq < p->cfg.queries_per_group) {
+      q = order[oi];
       const int need = qempty[q] ? 0 : qcols[q];
       if (g.nslots + need > slot_cap) break;
       g.qslot[g.nq] = (uint8_t)g.nslots;
@@ -728,7 +758,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       Q.empty = qempty[q];
       Q.id = q;
       g.nq++;
-      q++;
+      oi++;
     }
     g.qslot[g.nq] = (uint8_t)g.nslots;
     groups.push_back(g);
