@@ -1,0 +1,245 @@
+// bindex_driver.cpp -- command-line query driver over the engine's C ABI, with the command
+// line and the check output of the reference's GPU driver (bindex_cuda.cpp:1708-1995):
+//
+//   rts_bindex_driver [-f <data file>] [-b <columns>] [-p <query file>] [-n <rows>]
+//                     [-s <seed>] [-m single|batch]
+//
+// Data: the data file (columns of n little-endian uint32, one after another), or seeded
+// uniform columns when no file is given (the reference draws unseeded random data).  Its
+// row count comes from -n, or from the file size.  Every column gets a BinDex
+// (init_bindex_in_GPU).  Queries: the query file (default test/scan_cmd.txt, as there), one
+// line per column.  Each query runs as one fused sieve + refine + AND launch
+// (rts_bindex_query), or -m batch runs all of them in one batched launch; its bitmap is
+// copied to page-locked host memory and checked against raw_scan of the same predicates
+// with compare_bitmap, which prints "[CHECK]hit true/total" (and "[ERROR] ..." at the first
+// wrong row).  Exit status: 0 when every query matched, 1 on a mismatch, 2 on an error.
+#include <hip/hip_runtime_api.h>
+
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <unistd.h>
+
+#include <chrono>
+#include <string>
+#include <vector>
+
+#include "rts_batch.h"
+#include "rts_bindex.h"
+#include "rts_check.h"
+#include "rts_query.h"
+
+namespace {
+
+int fail(const char *what) {
+  fprintf(stderr, "[ERROR] %s: %s\n", what, rts_last_error());
+  return 2;
+}
+
+double ms_since(std::chrono::steady_clock::time_point t0) {
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+const char *op_name(int op) {
+  static const char *names[] = {"eq", "lt", "gt", "le", "ge", "bt"};
+  return op >= 0 && op <= 5 ? names[op] : "?";
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+  std::string data_path, query_path = "test/scan_cmd.txt", mode = "single";
+  int ncols = 1;
+  long long n = 0;
+  unsigned long long seed = 1;
+  int opt;
+  while ((opt = getopt(argc, argv, "hf:b:p:n:s:m:")) != -1) {
+    switch (opt) {
+      case 'h':
+        printf("Usage: %s [-f <input-file>] [-b <columns>] [-p <scan-cmd-file>] [-n <rows>] [-s <seed>] "
+               "[-m single|batch]\n", argv[0]);
+        return 0;
+      case 'f': data_path = optarg; break;
+      case 'b': ncols = atoi(optarg); break;
+      case 'p': query_path = optarg; break;
+      case 'n': n = atoll(optarg); break;
+      case 's': seed = strtoull(optarg, nullptr, 10); break;
+      case 'm': mode = optarg; break;
+      default:
+        printf("Error: unknown option %c\n", (char)opt);
+        return 2;
+    }
+  }
+  if (ncols < 1 || ncols > RTS_BATCH_MAX_COLS || (mode != "single" && mode != "batch")) {
+    fprintf(stderr, "[ERROR] need 1 <= -b <= %d and -m single|batch\n", RTS_BATCH_MAX_COLS);
+    return 2;
+  }
+  if (!data_path.empty() && n == 0) {
+    struct stat st;
+    if (stat(data_path.c_str(), &st) != 0) return fail("data file");
+    n = (long long)(st.st_size / (4 * ncols));
+  }
+  if (n < RTS_VAREA_N || n > 0x7FFFFFFFll) {
+    fprintf(stderr, "[ERROR] rows must be in [%d, 2^31): give -n or a data file\n", RTS_VAREA_N);
+    return 2;
+  }
+  printf("N = %lld\n", n);
+
+  // data (host), as the reference keeps it for its checker
+  std::vector<std::vector<CODE>> data(ncols, std::vector<CODE>((size_t)n));
+  std::vector<CODE *> cols(ncols);
+  for (int c = 0; c < ncols; c++) cols[c] = data[c].data();
+  if (!data_path.empty()) {
+    if (rts_data_file_read(data_path.c_str(), ncols, n, cols.data()) != 0) return fail("rts_data_file_read");
+  } else {
+    printf("initing data by random (seed %llu)\n", seed);
+    for (int c = 0; c < ncols; c++)
+      if (rts_generate_column(0, seed, c, (uint64_t)n, 0.0, 0.0, cols[c], 0) != 0) return fail("generate");
+  }
+
+  // indexes (bindex_cuda.cpp:1763-1780)
+  std::vector<BinDex> bx(ncols);
+  size_t free0 = 0, total = 0, free1 = 0, base_data = 0;
+  (void)hipMemGetInfo(&free0, &total);
+  auto t0 = std::chrono::steady_clock::now();
+  for (int c = 0; c < ncols; c++) {
+    printf("Build the bindex structure %d...\n", c);
+    size_t before = base_data;
+    init_bindex_in_GPU(&bx[c], cols[c], (POSTYPE)n, base_data);
+    if (base_data == before) return fail("init_bindex_in_GPU");
+  }
+  printf("[Time] Index Build Time: %.3f ms\n", ms_since(t0));
+  (void)hipMemGetInfo(&free1, &total);
+  printf("[Mem] Device memory used(MB): %.1f\n", (double)(free0 - free1 - base_data) / (1 << 20));
+
+  rts_query_file *qf = nullptr;
+  int bad_line = 0;
+  if (rts_query_file_load(query_path.c_str(), ncols, &qf, &bad_line) != 0) {
+    fprintf(stderr, "[ERROR] query file %s: line %d\n", query_path.c_str(), bad_line);
+    return 2;
+  }
+  const int nq = rts_query_file_count(qf);
+  printf("BinDex scan... (%d queries, %s)\n", nq, mode.c_str());
+
+  // closed ranges of every query
+  std::vector<rts_query_spec> specs(nq);
+  for (int q = 0; q < nq; q++) {
+    rts_query_spec &sp = specs[q];
+    memset(&sp, 0, sizeof(sp));
+    sp.ncols = ncols;
+    for (int c = 0; c < ncols; c++) {
+      const rts_column_predicate *p = rts_query_file_get(qf, q, c);
+      if (p->ntargets != 1) {
+        fprintf(stderr, "[ERROR] query %d column %d: lists of targets are not supported\n", q, c);
+        return 2;
+      }
+      sp.column[c] = c;
+      if (!rts_predicate_interval((enum OPERATOR)p->op, p->t1[0], p->t2 ? p->t2[0] : 0, &sp.lo[c], &sp.hi[c])) {
+        sp.lo[c] = 1;  // empty: lo > hi
+        sp.hi[c] = 0;
+      }
+      sp.mode[c] = RTS_EVAL_BINDEX;
+    }
+  }
+
+  const size_t words = (size_t)(n + 31) / 32, stride = (words + 3) / 4 * 4;
+  BITS *dev_out = nullptr, *h_result = nullptr;
+  unsigned long long *dev_count = nullptr;
+  const int nout = mode == "batch" ? (nq > 0 ? nq : 1) : 1;
+  if (rts_malloc((void **)&dev_out, stride * 4 * nout) != hipSuccess ||
+      rts_malloc((void **)&dev_count, 8 * nout) != hipSuccess ||
+      rts_host_alloc((void **)&h_result, stride * 4) != hipSuccess)
+    return fail("allocation");
+  hipStream_t stream = nullptr;
+  if (rts_stream_create(&stream) != hipSuccess) return fail("stream");
+
+  rts_batch_plan *plan = nullptr;
+  double batch_ms = 0;
+  if (mode == "batch" && nq > 0) {
+    std::vector<const CODE *> dcols(ncols);
+    std::vector<const BinDex *> idx(ncols);
+    for (int c = 0; c < ncols; c++) {
+      dcols[c] = bx[c].rawDataInGPU;
+      idx[c] = &bx[c];
+    }
+    if (rts_batch_plan_create(&plan, dcols.data(), idx.data(), ncols, n, nq, nullptr, stream) != hipSuccess ||
+        rts_batch_plan_set_queries(plan, specs.data(), nq, stream) != hipSuccess)
+      return fail("batch plan");
+    auto tb = std::chrono::steady_clock::now();
+    if (rts_batch_plan_launch(plan, dev_out, (int64_t)stride, dev_count, stream) != hipSuccess ||
+        rts_stream_synchronize(stream) != hipSuccess)
+      return fail("batch launch");
+    batch_ms = ms_since(tb);
+    printf("[Time] batched launch of %d queries: %.3f ms\n", nq, batch_ms);
+  }
+
+  std::vector<BITS> check(words), colbm(words);
+  int mismatches = 0;
+  for (int q = 0; q < nq; q++) {
+    for (int c = 0; c < ncols; c++) {
+      const rts_column_predicate *p = rts_query_file_get(qf, q, c);
+      if (p->op == BT)
+        printf("%s %u %u\n", op_name(p->op), p->t1[0], p->t2[0]);
+      else
+        printf("%s %u\n", op_name(p->op), p->t1[0]);
+    }
+    BITS *res = dev_out;
+    double scan_ms = 0;
+    if (mode == "single") {
+      std::vector<const BinDex *> idx(ncols);
+      for (int c = 0; c < ncols; c++) idx[c] = &bx[c];
+      auto ts = std::chrono::steady_clock::now();
+      if (rts_bindex_query(idx.data(), specs[q].lo, specs[q].hi, ncols, dev_out, dev_count, stream) != hipSuccess ||
+          rts_stream_synchronize(stream) != hipSuccess)
+        return fail("rts_bindex_query");
+      scan_ms = ms_since(ts);
+    } else {
+      res = dev_out + (size_t)q * stride;
+    }
+    auto tt = std::chrono::steady_clock::now();
+    if (rts_memcpy_dtoh_async(h_result, res, words * 4, stream) != hipSuccess ||
+        rts_stream_synchronize(stream) != hipSuccess)
+      return fail("result transfer");
+    const double xfer_ms = ms_since(tt);
+    if (mode == "single") printf("[Time] query %d: scan %.3f ms, transfer %.3f ms\n", q, scan_ms, xfer_ms);
+
+    // ground truth: raw_scan of every column (bindex.cpp:1851-1892, the bt swap of
+    // raw_scan_entry), ANDed across columns
+    printf("[CHECK]check final result.\n");
+    for (int c = 0; c < ncols; c++) {
+      const rts_column_predicate *p = rts_query_file_get(qf, q, c);
+      CODE a = p->t1[0], b = p->t2 ? p->t2[0] : 0;
+      if (p->op == BT && a > b) std::swap(a, b);
+      std::fill(colbm.begin(), colbm.end(), 0u);
+      raw_scan(&bx[c], colbm.data(), a, b, (enum OPERATOR)p->op, cols[c], nullptr);
+      for (size_t w = 0; w < words; w++) check[w] = c ? check[w] & colbm[w] : colbm[w];
+    }
+    rts_compare_result r{};
+    compare_bitmap(check.data(), h_result, (int)n, cols.data(), ncols);
+    // the same comparison, counted, for the exit status
+    BITS *dcheck = nullptr;
+    if (rts_malloc((void **)&dcheck, words * 4) != hipSuccess ||
+        rts_memcpy_htod(dcheck, check.data(), words * 4) != hipSuccess ||
+        rts_bitmap_compare(dcheck, res, n, &r, stream) != hipSuccess)
+      return fail("rts_bitmap_compare");
+    (void)rts_free(dcheck);
+    if (r.first_mismatch >= 0) mismatches++;
+    printf("[CHECK]check final result done.\n\n");
+    for (size_t w = 0; w < words; w++)
+      if (h_result[w] != 0) {
+        printf("[CHECK] not all 0!\n");
+        break;
+      }
+  }
+  printf("[CHECK] %d of %d queries matched raw_scan\n", nq - mismatches, nq);
+
+  if (plan) rts_batch_plan_free(plan);
+  (void)rts_stream_destroy(stream);
+  (void)rts_host_free(h_result);
+  (void)rts_free(dev_out);
+  (void)rts_free(dev_count);
+  rts_query_file_free(qf);
+  for (int c = 0; c < ncols; c++) rts_bindex_free(&bx[c]);
+  return mismatches ? 1 : 0;
+}

@@ -125,3 +125,15 @@ def test_generator_rejects_bad_parameters():
         R.generate_column(R.DIST_ZIPF, 1, 0, 10, -1.0, 100)
     with pytest.raises(ValueError):
         R.generate_column(7, 1, 0, 10)
+
+
+def test_query_driver_is_built_and_prints_usage():
+    """The reference-style query driver links against the library and parses its command
+    line without a GPU (-h prints the usage and exits 0; a bad column count exits 2)."""
+    import subprocess
+    drv = os.path.join(ROOT, "rtscan_amd", "rts_bindex_driver")
+    assert os.path.isfile(drv)
+    p = subprocess.run([drv, "-h"], capture_output=True, text=True, timeout=30)
+    assert p.returncode == 0 and "-p <scan-cmd-file>" in p.stdout
+    p = subprocess.run([drv, "-b", "9", "-n", "1000"], capture_output=True, text=True, timeout=30)
+    assert p.returncode == 2
