@@ -40,9 +40,10 @@ struct QueryArgs {
   BoundDev hi[C];  // L(b + 1)
 };
 
-__device__ __forceinline__ void mark_removed(const BoundDev &b, int64_t t, uint32_t *tog) {
-  if (!b.dir) return;
-  const uint32_t e0 = b.dir[t], e1 = b.dir[t + 1];
+// Rows of bound b's refine list in tile t with value >= thr: marked in the tile's LDS
+// toggles.  The list's tile slice [e0, e1) was read from the directory beforehand.
+__device__ __forceinline__ void mark_removed(const BoundDev &b, uint32_t e0, uint32_t e1, int64_t t,
+                                             uint32_t *tog) {
   const uint32_t row0 = (uint32_t)(t * kTileRows);
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const uint2 pv = b.bent[e];
@@ -71,21 +72,42 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
   const uint32_t last_mask = tail_mask((uint64_t)n);
   uint32_t pc = 0;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    // The filter-vector words and the lists' directory words do not depend on the refine:
+    // issue them first, so that one round trip covers both and the second round trip is
+    // the refine entries.
+    const int lw = threadIdx.x * kPer;
+    const int64_t w = t * kTileWords + lw;
+    uint4 lo_w[C], hi_w[C];
+    uint32_t lo_e[C][2], hi_e[C][2];
+#pragma unroll
+    for (int c = 0; c < C; c++) {
+      if (w < words) {
+        lo_w[c] = base_words(q.lo[c], w, words);
+        hi_w[c] = base_words(q.hi[c], w, words);
+      }
+      lo_e[c][0] = lo_e[c][1] = hi_e[c][0] = hi_e[c][1] = 0;
+      if (q.lo[c].dir) {
+        lo_e[c][0] = q.lo[c].dir[t];
+        lo_e[c][1] = q.lo[c].dir[t + 1];
+      }
+      if (q.hi[c].dir) {
+        hi_e[c][0] = q.hi[c].dir[t];
+        hi_e[c][1] = q.hi[c].dir[t + 1];
+      }
+    }
     for (int i = threadIdx.x; i < C * 2 * kTileWords; i += kBlock) (&tog[0][0][0])[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < C; c++) {
-      mark_removed(q.lo[c], t, tog[c][0]);
-      mark_removed(q.hi[c], t, tog[c][1]);
+      mark_removed(q.lo[c], lo_e[c][0], lo_e[c][1], t, tog[c][0]);
+      mark_removed(q.hi[c], hi_e[c][0], hi_e[c][1], t, tog[c][1]);
     }
     __syncthreads();
-    const int lw = threadIdx.x * kPer;
-    const int64_t w = t * kTileWords + lw;
     if (w < words) {
       uint4 acc = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        uint4 lo = base_words(q.lo[c], w, words), hi = base_words(q.hi[c], w, words);
+        const uint4 lo = lo_w[c], hi = hi_w[c];
         const uint32_t *tl = &tog[c][0][lw], *th = &tog[c][1][lw];
         acc.x &= (hi.x ^ th[0]) & ~(lo.x ^ tl[0]);
         acc.y &= (hi.y ^ th[1]) & ~(lo.y ^ tl[1]);
