@@ -10,14 +10,16 @@
 // every query of the group.  Items are assigned statically so that the items of one tile
 // run on the same XCD one after another (workgroup b runs on XCD b % 8 on MI355X, and XCD x
 // takes tiles t with t % 8 == x, tile-major): the tile's filter-vector and list slices are
-// read from HBM once and re-read by the other query groups from that XCD's L2.  Popcounts
-// go to one partial per (query, tile), summed by a second small kernel.
+// read from HBM once and, as far as the 4 MB L2 holds them, re-read by the other query
+// groups from there.  Popcounts go to one partial per (query, tile), summed by a second
+// small kernel.
 //
-// The item is latency-bound unless many loads are in flight, so each phase issues its
-// loads together: the group's slot and list descriptors are flattened at plan time (one
-// load each, then the two directory words), refine entries are taken 8 per thread with
-// all 16 loads issued before the LDS toggles, and every slot's filter-vector words are
-// loaded for 4 slots at a time.
+// An item is latency-bound unless many loads are in flight, so each phase issues its loads
+// together: a group is one contiguous record (slots, lists, queries) loaded in one round,
+// then the lists' directory words; refine entries are list-major (wave v owns lists v,
+// v + 8, ...; a list's slice base is wave-uniform), 16 entries per lane in flight; the
+// filter-vector words go out with the first refine round and are XORed into the LDS
+// toggles as they arrive.
 #include <string.h>
 
 #include <algorithm>
@@ -242,16 +244,14 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
     const int nslots = g->nslots, nq = g->nq;
     __syncthreads();
 
-    // 2. filter-vector words of every BinDex slot (loads issued first), then the refine
-    //    toggles, several entries per thread with their loads issued together
     const int64_t w = t * TW + lw;
-    // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time (their
-    //    pointers come from the group record through scalar loads), XORed into the toggles
-    //    as they arrive: every toggle update is an XOR, so the order does not matter and the
-    //    words need not stay in registers.  Refine entries: wave v owns lists v, v + 8, ...;
-    //    a list's slice base is uniform across the wave, each lane takes kListUnroll entries
-    //    of each of its lists per round, and the first round's loads go out together with
-    //    the first 8 slots' filter-vector loads.
+    // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time
+    //    (16 at 512-word tiles; their pointers come from the group record through scalar
+    //    loads), XORed into the toggles as they arrive: every toggle update is an XOR, so
+    //    the order does not matter and the words need not stay in registers.  Refine
+    //    entries: wave v owns lists v, v + 8, ...; a list's slice base is uniform across the
+    //    wave, each lane takes kListUnroll entries of each of its lists per round, and the
+    //    first round's loads go out together with the filter-vector loads.
     constexpr int kListsPerWave = kLists / kBatchWaves;
     constexpr int kListUnroll = 16 / kListsPerWave;
     static_assert(kLists % kBatchWaves == 0, "lists per wave");
