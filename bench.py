@@ -10,9 +10,12 @@ stream, bracketed by HIP events on that stream.
 Prints ONE JSON line: rows scanned per second, the dominant kernel's HBM roofline
 fraction (algorithmic bytes per launch / measured time per launch), and the CPU baseline
 (the oracle's multithreaded port of the reference raw_scan, timed on this host on a
-bounded sample of the same predicates).
+bounded sample of the same predicates).  `extra` carries the other configurations: the
+BinDex query of the same predicates, conjunctions over the 1e8 x 4 table, 4096 batched
+queries, and the request loop over resident 2^26 x 2 and 1e9 x 4 tables (p50 / p99).
 """
 import argparse
+import gc
 import json
 import os
 import sys
@@ -542,7 +545,7 @@ def main():
                     help="tile_words,queries_per_group,blocks_per_cu[,order] for the batch "
                          "workload (skips the autotuner)")
     ap.add_argument("--no-extra", action="store_true",
-                    help="skip the conjunction and batched legs of the default run")
+                    help="skip the conjunction, batched and serving legs of the default run")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
@@ -578,6 +581,11 @@ def main():
                 del table
             except Exception as e:  # noqa: BLE001
                 res["extra"]["error"] = "%s: %s" % (type(e).__name__, e)
+            gc.collect()  # the 1e8 x 4 table and the batch outputs leave HBM before serving
+            try:
+                res["extra"]["serve"] = bench_serve(R, n2=args.serve_rows)
+            except Exception as e:  # noqa: BLE001
+                res["extra"]["serve_error"] = "%s: %s" % (type(e).__name__, e)
     print(json.dumps(res), flush=True)
 
 
