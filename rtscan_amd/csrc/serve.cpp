@@ -121,7 +121,7 @@ struct rts_server {
   std::unordered_map<uint64_t, Request> reqs;
   std::deque<uint64_t> queue;
   uint64_t next_ticket = 1;
-  std::mutex mu;
+  mutable std::mutex mu;
   std::condition_variable cv_loop, cv_done;
   std::thread loop;
   std::atomic<bool> running{false};
@@ -153,17 +153,34 @@ void complete_batch(rts_server *s, Worker &w, uint64_t t_done) {
   w.busy = false;
 }
 
+// A batch whose work reported an error: its requests fail and their bitmaps return to the
+// pool (replayed ones count as completed, so a replay ends).
+void fail_batch(rts_server *s, Worker &w) {
+  for (uint64_t t : w.batch) {
+    auto it = s->reqs.find(t);
+    if (it == s->reqs.end()) continue;
+    Request &r = it->second;
+    r.state = RTS_REQ_FAILED;
+    s->pool.release(r.off, r.bytes);
+    r.off = SIZE_MAX;
+    if (r.auto_release) s->reqs.erase(it);
+    s->completed++;
+  }
+  w.batch.clear();
+  w.busy = false;
+}
+
 // Admit the oldest waiting request's table's requests (in arrival order) into one batch on
 // worker w.  Returns true when a batch was launched.
 bool launch_batch(rts_server *s, Worker &w) {
   if (s->queue.empty()) return false;
-  const int table = s->reqs[s->queue.front()].table;
+  const int table = s->reqs.at(s->queue.front()).table;
   const Table &tb = s->tables[table];
   std::vector<uint64_t> admitted;
   std::vector<rts_query_spec> specs;
   std::vector<BITS *> outs;
   for (auto it = s->queue.begin(); it != s->queue.end() && (int)admitted.size() < s->cfg.max_batch;) {
-    Request &r = s->reqs[*it];
+    Request &r = s->reqs.at(*it);
     if (r.table != table) {
       ++it;
       continue;
@@ -189,7 +206,7 @@ bool launch_batch(rts_server *s, Worker &w) {
   if (e != hipSuccess) {
     s->loop_error = e;
     for (uint64_t t : admitted) {
-      Request &r = s->reqs[t];
+      Request &r = s->reqs.at(t);
       r.state = RTS_REQ_FAILED;
       s->pool.release(r.off, r.bytes);
       r.off = SIZE_MAX;
@@ -210,8 +227,14 @@ void loop_main(rts_server *s) {
   while (s->running) {
     bool progress = false;
     for (Worker &w : s->workers) {
-      if (w.busy && hipEventQuery(w.done) == hipSuccess) {
+      if (!w.busy) continue;
+      const hipError_t q = hipEventQuery(w.done);
+      if (q == hipSuccess) {
         complete_batch(s, w, now_ns());
+        progress = true;
+      } else if (q != hipErrorNotReady) {  // the batch's work failed: fail its requests
+        s->loop_error = q;
+        fail_batch(s, w);
         progress = true;
       }
     }
@@ -281,12 +304,18 @@ hipError_t rts_server_add_table(rts_server *s, const CODE *const *columns, const
   for (int c = 0; indexes && c < ncols; c++) t.indexes[c] = indexes[c];
   t.n = n;
   t.bitmap_bytes = (size_t)((n + 31) / 32) * 4;
-  for (Worker &w : s->workers) {
+  for (size_t wi = 0; wi < s->workers.size(); wi++) {
     rts_batch_plan *p = nullptr;
     hipError_t e = rts_batch_plan_create(&p, t.columns.data(), t.indexes.data(), ncols, n,
-                                         s->cfg.max_batch, &s->cfg.batch, w.stream);
-    if (e != hipSuccess) return e;
-    w.plans.push_back(p);
+                                         s->cfg.max_batch, &s->cfg.batch, s->workers[wi].stream);
+    if (e != hipSuccess) {  // keep every worker's plan list aligned with the table list
+      for (size_t k = 0; k < wi; k++) {
+        rts_batch_plan_free(s->workers[k].plans.back());
+        s->workers[k].plans.pop_back();
+      }
+      return e;
+    }
+    s->workers[wi].plans.push_back(p);
   }
   s->tables.push_back(std::move(t));
   if (table_id) *table_id = (int)s->tables.size() - 1;
@@ -370,6 +399,7 @@ void rts_server_release(rts_server *s, uint64_t ticket) {
 void rts_server_stats(const rts_server *s, uint64_t *submitted, uint64_t *completed, uint64_t *batches,
                       int *max_batch_seen, size_t *pool_in_use, size_t *pool_peak) {
   if (!s) return;
+  std::lock_guard<std::mutex> lock(s->mu);
   if (submitted) *submitted = s->submitted;
   if (completed) *completed = s->completed;
   if (batches) *batches = s->batches;
@@ -412,6 +442,7 @@ hipError_t rts_server_replay(rts_server *s, const int *table_ids, const rts_quer
   const uint64_t done0 = s->completed;
   const uint64_t t0 = now_ns();
   hipError_t err = hipSuccess;
+  int sent = 0;  // requests actually submitted: the wait below counts only those
   std::thread submitter([&] {
     for (int i = 0; i < nq; i++) {
       const uint64_t due = t0 + arrival_ns[i];
@@ -427,12 +458,13 @@ hipError_t rts_server_replay(rts_server *s, const int *table_ids, const rts_quer
         err = e;
         return;
       }
+      sent++;
     }
   });
   submitter.join();
   {
     std::unique_lock<std::mutex> lock(s->mu);
-    s->cv_done.wait(lock, [&] { return s->completed - done0 >= (uint64_t)nq || s->loop_error != hipSuccess; });
+    s->cv_done.wait(lock, [&] { return s->completed - done0 >= (uint64_t)sent || s->loop_error != hipSuccess; });
   }
   if (elapsed_s) *elapsed_s = (now_ns() - t0) * 1e-9;
   return err != hipSuccess ? err : s->loop_error;
