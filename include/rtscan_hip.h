@@ -17,6 +17,9 @@
  *     forcing device 0 (cuda_and.cu:109).  rts_* entry points are asynchronous on the
  *     stream they are given.
  *   - Errors are returned, never fatal: no entry point exits the process.
+ *   - Entry points may be called from several host threads at once, also on one stream
+ *     (the reference calls its wrappers from per-column threads, bindex_cuda.cpp:1901-1914):
+ *     internal scratch is kept per (device, stream, host thread).
  */
 #ifndef RTSCAN_HIP_H_
 #define RTSCAN_HIP_H_

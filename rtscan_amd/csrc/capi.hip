@@ -59,10 +59,11 @@ uint32_t *stream_count_partials(hipStream_t stream) {
   static std::map<std::tuple<int, uintptr_t, std::thread::id>, uint32_t *> table;
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  // hipStreamPerThread names a different stream on every host thread
-  const std::thread::id tid =
-      stream == hipStreamPerThread ? std::this_thread::get_id() : std::thread::id();
-  const auto key = std::make_tuple(dev, (uintptr_t)stream, tid);
+  // One buffer per (device, stream, host thread): two threads enqueueing counted work on the
+  // same stream (the legacy null stream, shared by all threads, or hipStreamPerThread,
+  // which names a different stream on every thread) must not share partials, since their
+  // launches can interleave between a scan and its finalize.
+  const auto key = std::make_tuple(dev, (uintptr_t)stream, std::this_thread::get_id());
   std::lock_guard<std::mutex> lock(mu);
   auto it = table.find(key);
   if (it != table.end()) return it->second;

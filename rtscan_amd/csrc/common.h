@@ -96,7 +96,8 @@ __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink)
   }
 }
 
-// Scratch of (current device, stream) for count partials, allocated on first use.
+// Scratch of (current device, stream, calling host thread) for count partials, allocated on
+// first use.
 uint32_t *stream_count_partials(hipStream_t stream);
 
 // Sum partials[0, nblocks) into *out on `stream` (one small launch).

@@ -250,3 +250,38 @@ def test_config2_scale_plain_scan(eng):
         truth = O.raw_scan(x, op, t1, t2, threads=16)
         assert np.array_equal(got, truth)
         assert read_count(eng, cnt) == O.popcount_rows(truth, n)
+
+
+def test_counted_scans_from_several_threads_on_one_stream(eng):
+    """Host threads that count on the same (null) stream at the same time each get their
+    own count scratch: every count is right (the reference calls its wrappers from one host
+    thread per column, bindex_cuda.cpp:1901-1914)."""
+    import threading
+    n = 1 << 20
+    rng = np.random.default_rng(8)
+    cols = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32) for _ in range(4)]
+    errors = []
+
+    def worker(k):
+        try:
+            d = eng.DeviceBuffer.from_numpy(cols[k])
+            out = eng.DeviceBuffer(4 * (n // 32))
+            cnt = eng.DeviceBuffer(8)
+            r = np.random.default_rng(k)
+            for _ in range(60):
+                lo, hi = sorted(int(v) for v in r.integers(0, 2 ** 32, 2))
+                eng.scan_range(d, n, lo, hi, out, cnt)  # null stream, synchronous per call
+                eng.synchronize()
+                got = int(cnt.to_numpy(np.uint64, 1)[0])
+                want = int(((cols[k] >= lo) & (cols[k] <= hi)).sum())
+                if got != want:
+                    errors.append((k, lo, hi, got, want))
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors[:3]
