@@ -11,11 +11,12 @@
  *   - Bitmaps are 1 bit per row, MSB-first inside 32-bit words: row i is bit
  *     (1u << (31 - i % 32)) of word i / 32 (bindex.cpp:269).  rts_* entry points write
  *     ceil(n / 32) words and leave the bits past row n at 0.
- *   - Reference-named wrappers run on the legacy null stream and return after the work
- *     has finished (the reference ends every wrapper with cudaDeviceSynchronize,
- *     cuda_and.cu:276-287).  They use the calling thread's current device instead of
- *     forcing device 0 (cuda_and.cu:109).  rts_* entry points are asynchronous on the
- *     stream they are given.
+ *   - Reference-named wrappers run on the calling thread's default stream
+ *     (hipStreamPerThread: ordered after work on the legacy null stream) and return after
+ *     the work has finished (the reference ends every wrapper with cudaDeviceSynchronize,
+ *     cuda_and.cu:276-287); calls from different host threads may overlap on the device.
+ *     They use the calling thread's current device instead of forcing device 0
+ *     (cuda_and.cu:109).  rts_* entry points are asynchronous on the stream they are given.
  *   - Errors are returned, never fatal: no entry point exits the process.
  *   - Entry points may be called from several host threads at once, also on one stream
  *     (the reference calls its wrappers from per-column threads, bindex_cuda.cpp:1901-1914):

@@ -275,7 +275,7 @@ static hipError_t refine_slots(BinDex **bxs, int ncols, int sel, const CODE *pre
   const int64_t items = s1 - s0;
   const unsigned grid =
       (unsigned)std::min<int64_t>((items + kBlock - 1) / kBlock, (int64_t)device_cu_count() * 16);
-  hipLaunchKernelGGL(k_refine_slots, dim3(grid), dim3(kBlock), 0, 0, rc, ncols,
+  hipLaunchKernelGGL(k_refine_slots, dim3(grid), dim3(kBlock), 0, ref_stream(), rc, ncols,
                      bxs[sel]->dev->sorted_pos, s0, s1, bitmap, inverse);
   return hipGetLastError();
 }
@@ -330,12 +330,12 @@ void copy_filter_vector_in_GPU(BinDex *bindex, BITS *dev_bitmap, int k, bool neg
   const uint64_t words = ref_words(bindex->length);
   hipError_t e;
   if (k < 0)
-    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0u, 0);
+    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0u, ref_stream());
   else if (k >= kBins - 1)
-    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0xFFFFFFFFu, 0);
+    e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0xFFFFFFFFu, ref_stream());
   else
-    e = bitop(negation ? 1 : 0, dev_bitmap, bindex->filterVectorsInGPU[k], nullptr, words, 0u, 0);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+    e = bitop(negation ? 1 : 0, dev_bitmap, bindex->filterVectorsInGPU[k], nullptr, words, 0u, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   (void)set_error(e, "copy_filter_vector_in_GPU");
 }
 
@@ -370,7 +370,7 @@ hipError_t GPURefineAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CODE
   }
   hipError_t e = refine_slots(bindexs, column_num, selected_id, predicate, s0, s1,
                               dev_result_bitmap, inverse);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   return set_error(e, "GPURefineAreaWithCuda");
 }
 
@@ -397,7 +397,7 @@ hipError_t GPURefineEqAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CO
   const int64_t s1 = block_end_slot(b->areas[ar], br);
   hipError_t e = refine_slots(bindexs, column_num, selected_id, predicate, s0, s1,
                               dev_result_bitmap, inverse);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   return set_error(e, "GPURefineEqAreaWithCuda");
 }
 
@@ -422,8 +422,8 @@ hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYP
   CODE *d = nullptr;
   hipError_t e = hipMalloc((void **)&d, (size_t)n * sizeof(CODE));
   if (e == hipSuccess) e = hipMemcpy(d, host_column, (size_t)n * sizeof(CODE), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = bindex_build(bindex, d, n, d, 0);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = bindex_build(bindex, d, n, d, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e != hipSuccess && d) (void)hipFree(d);
   return set_error(e, "rts_bindex_build_host");
 }

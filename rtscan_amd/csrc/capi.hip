@@ -113,7 +113,7 @@ hipError_t launch_count_finalize(const CountSink &sink, unsigned nblocks, hipStr
 // cudaDeviceSynchronize); launch errors are returned, never fatal.
 static hipError_t finish_sync(hipError_t e, const char *what) {
   if (e != hipSuccess) return set_error(e, what);
-  return set_error(hipStreamSynchronize(0), what);
+  return set_error(hipStreamSynchronize(ref_stream()), what);
 }
 
 }  // namespace rts
@@ -125,28 +125,28 @@ extern "C" {
 // ---------------------------------------------------------------- cuda_and.cu surface
 
 hipError_t GPUbitAndWithCuda(BITS *a, BITS *b, unsigned int n) {
-  return finish_sync(bitop(kAnd, a, b, nullptr, ref_words((int)n), 0, 0), "GPUbitAndWithCuda");
+  return finish_sync(bitop(kAnd, a, b, nullptr, ref_words((int)n), 0, ref_stream()), "GPUbitAndWithCuda");
 }
 
 hipError_t GPUbitOrWithCuda(BITS *a, BITS *b, unsigned int n) {
-  return finish_sync(bitop(kOr, a, b, nullptr, ref_words((int)n), 0, 0), "GPUbitOrWithCuda");
+  return finish_sync(bitop(kOr, a, b, nullptr, ref_words((int)n), 0, ref_stream()), "GPUbitOrWithCuda");
 }
 
 hipError_t GPUbitCopyWithCuda(BITS *a, BITS *b, unsigned int n) {
-  return finish_sync(bitop(kCopy, a, b, nullptr, n, 0, 0), "GPUbitCopyWithCuda");
+  return finish_sync(bitop(kCopy, a, b, nullptr, n, 0, ref_stream()), "GPUbitCopyWithCuda");
 }
 
 hipError_t GPUbitCopyNegationWithCuda(BITS *a, BITS *b, unsigned int n) {
-  return finish_sync(bitop(kNot, a, b, nullptr, n, 0, 0), "GPUbitCopyNegationWithCuda");
+  return finish_sync(bitop(kNot, a, b, nullptr, n, 0, ref_stream()), "GPUbitCopyNegationWithCuda");
 }
 
 // result = b & ~a (bitmap_copySIMDKernel, cuda_and.cu:54-61)
 hipError_t GPUbitCopySIMDWithCuda(BITS *result, BITS *a, BITS *b, unsigned int bitnum) {
-  return finish_sync(bitop(kAndNot, result, a, b, bitnum, 0, 0), "GPUbitCopySIMDWithCuda");
+  return finish_sync(bitop(kAndNot, result, a, b, bitnum, 0, ref_stream()), "GPUbitCopySIMDWithCuda");
 }
 
 hipError_t GPUbitMemsetWithCuda(BITS *a, BITS value, unsigned int n) {
-  return finish_sync(bitop(kFill, a, nullptr, nullptr, ref_words((int)n), value, 0),
+  return finish_sync(bitop(kFill, a, nullptr, nullptr, ref_words((int)n), value, ref_stream()),
                      "GPUbitMemsetWithCuda");
 }
 
@@ -158,8 +158,8 @@ hipError_t CPUbitAndWithCuda(BITS *host_a, BITS *host_b, unsigned int size) {
   if (e == hipSuccess) e = hipMalloc((void **)&db, bytes);
   if (e == hipSuccess) e = hipMemcpy(da, host_a, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(db, host_b, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = bitop(kAnd, da, db, nullptr, words, 0, 0);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = bitop(kAnd, da, db, nullptr, words, 0, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e == hipSuccess) e = hipMemcpy(host_a, da, bytes, hipMemcpyDeviceToHost);
   if (da) (void)hipFree(da);
   if (db) (void)hipFree(db);
@@ -172,8 +172,8 @@ hipError_t refineWithCuda(CODE *bitmap, unsigned int size) {
   BITS *d = nullptr;
   hipError_t e = hipMalloc((void **)&d, bytes);
   if (e == hipSuccess) e = hipMemcpy(d, bitmap, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = flip_rows(d, size, 0);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = flip_rows(d, size, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e == hipSuccess) e = hipMemcpy(bitmap, d, bytes, hipMemcpyDeviceToHost);
   if (d) (void)hipFree(d);
   return set_error(e, "refineWithCuda");
@@ -264,8 +264,8 @@ void raw_scan(struct BinDex *bindex, BITS *bitmap, CODE t1, CODE t2, enum OPERAT
   if (e == hipSuccess) e = hipMemcpy(dcol, raw_data, (size_t)n * sizeof(CODE), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(dbm, bitmap, words * sizeof(BITS), hipMemcpyHostToDevice);
   const CODE *cols[1] = {dcol};
-  if (e == hipSuccess) e = scan_conjunction(cols, &lo, &hi, 1, (uint64_t)n, dbm, nullptr, kStoreXor, 0);
-  if (e == hipSuccess) e = hipStreamSynchronize(0);
+  if (e == hipSuccess) e = scan_conjunction(cols, &lo, &hi, 1, (uint64_t)n, dbm, nullptr, kStoreXor, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e == hipSuccess) e = hipMemcpy(bitmap, dbm, words * sizeof(BITS), hipMemcpyDeviceToHost);
   if (dcol) (void)hipFree(dcol);
   if (dbm) (void)hipFree(dbm);

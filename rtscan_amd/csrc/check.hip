@@ -226,7 +226,7 @@ extern "C" void compare_bitmap(BITS *bitmap_a, BITS *bitmap_b, int len, CODE **r
   if (e == hipSuccess) e = hipMalloc((void **)&db, bytes);
   if (e == hipSuccess) e = hipMemcpy(da, bitmap_a, bytes, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(db, bitmap_b, bytes, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = rts::bitmap_compare(da, db, len, &r, 0);
+  if (e == hipSuccess) e = rts::bitmap_compare(da, db, len, &r, rts::ref_stream());
   (void)hipFree(da);
   (void)hipFree(db);
   if (e != hipSuccess) {

@@ -96,6 +96,13 @@ __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink)
   }
 }
 
+// Stream of the reference-named wrappers: the calling thread's default stream.  It is a
+// blocking stream (ordered after work on the legacy null stream, as the reference's
+// wrappers are), and calls from different host threads can run at the same time.  Every
+// wrapper synchronises it before returning, as the reference ends with
+// cudaDeviceSynchronize (cuda_and.cu:276-287).
+inline hipStream_t ref_stream() { return hipStreamPerThread; }
+
 // Scratch of (current device, stream, calling host thread) for count partials, allocated on
 // first use.
 uint32_t *stream_count_partials(hipStream_t stream);

@@ -285,3 +285,38 @@ def test_counted_scans_from_several_threads_on_one_stream(eng):
     for t in ts:
         t.join()
     assert not errors, errors[:3]
+
+
+def test_reference_wrappers_from_several_threads(eng):
+    """The reference-named wrappers run on each calling thread's default stream and return
+    after their work: several host threads calling them at once (as the reference's driver
+    does, one thread per column, bindex_cuda.cpp:1901-1914) each get their own right
+    result."""
+    import threading
+    L = eng.lib()
+    n = 1 << 20
+    W = n // 32
+    errors = []
+
+    def worker(k):
+        try:
+            r = np.random.default_rng(100 + k)
+            for _ in range(30):
+                a = r.integers(0, 2 ** 32, W, dtype=np.uint64).astype(np.uint32)
+                b = r.integers(0, 2 ** 32, W, dtype=np.uint64).astype(np.uint32)
+                da, db = eng.DeviceBuffer.from_numpy(a), eng.DeviceBuffer.from_numpy(b)
+                eng.check(L.GPUbitAndWithCuda(da.ptr, db.ptr, n), "and")
+                if not np.array_equal(da.to_numpy(np.uint32, W), a & b):
+                    errors.append(("and", k))
+                eng.check(L.GPUbitCopyNegationWithCuda(db.ptr, da.ptr, W), "not")
+                if not np.array_equal(db.to_numpy(np.uint32, W), ~(a & b)):
+                    errors.append(("not", k))
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    ts = [threading.Thread(target=worker, args=(k,)) for k in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors[:3]
