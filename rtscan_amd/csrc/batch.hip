@@ -48,13 +48,15 @@ struct BatchSlot {
 };
 
 // One refine list of a group: the rows of a bin's B list (in row order, tile directory
-// `dir`) with value >= thr flip bits of slot `slot`.  Unused entries point at a zero
+// `dir`) whose value lies in [thr, thr + span] flip bits of slot `slot` (span = 2^32 - 1 -
+// thr for a bound, value >= thr; the whole range of a narrow column, both of whose
+// bounds fall in this bin, in one pass over the list).  Unused entries point at a zero
 // directory (no rows), so the kernel loads every entry without first reading the count.
 struct BatchList {
   const uint32_t *dir;
   const uint2 *bent;  // (row, value) entries of the column's B lists
-  CODE thr;
-  uint32_t slot;
+  CODE thr, span;
+  uint32_t slot, pad;
 };
 
 // One query of a group: its own bitmap (or out + id * out_stride of the launch) and its
@@ -85,6 +87,7 @@ struct BatchLds {
   const uint2 *lent[kLists];  // first entry of the list's tile slice
   uint32_t llen[kLists];      // entries in the slice
   CODE lthr[kLists];
+  CODE lspan[kLists];
   uint32_t lslot[kLists];
   BatchSlot slot[kSlots];
   BITS *qout[kBatchMaxGroup];
@@ -229,6 +232,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
       lds.lent[tid] = L.bent + st;
       lds.llen[tid] = L.dir[t + 1] - st;
       lds.lthr[tid] = L.thr;
+      lds.lspan[tid] = L.span;
       lds.lslot[tid] = L.slot;
     } else if (tid >= kWave && tid < kWave + kSlots) {
       lds.slot[tid - kWave] = g->slot[tid - kWave];
@@ -306,13 +310,13 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
 #pragma unroll
       for (int i = 0; i < kListsPerWave; i++) {
         const int L = wv + i * kBatchWaves;
-        const uint32_t thr = lds.lthr[L];
+        const uint32_t thr = lds.lthr[L], span = lds.lspan[L];
         uint32_t *tg = lds.tog[lds.lslot[L]];
 #pragma unroll
         for (int u = 0; u < kListUnroll; u++) {
           const uint32_t e = off + u * kWave + lane;
           // rows of the slice lie in [row0, row0 + TW * 32): the word index is < TW
-          if (e < llen[i] && ent[i][u].y >= thr)
+          if (e < llen[i] && ent[i][u].y - thr <= span)
             atomicXor(&tg[(ent[i][u].x - row0) >> 5], 0x80000000u >> (ent[i][u].x & 31));
         }
       }
@@ -396,6 +400,8 @@ struct PlannedBound {
 struct PlannedCol {
   int col = 0;
   bool direct = false;
+  bool narrow = false;  // both bounds in one bin: one list pass over [lo_v, hi_v]
+  CODE lo_v = 0, hi_v = 0;
   CODE dlo = 0, dspan = 0;
   PlannedBound lo, hi;
 };
@@ -689,6 +695,9 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       if (sl.fv == sh.fv && sl.list >= 0 && sl.list == sh.list) {
         P.lo.base = P.hi.base = kBaseOnes;
         P.lo.fv = P.hi.fv = nullptr;
+        P.narrow = true;
+        P.lo_v = lo;
+        P.hi_v = hi;
         st_narrow++;
       }
       st_bindex++;
@@ -742,13 +751,23 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
           S.hibase = P.hi.base;
           S.lobase = P.lo.base;
           const rts_bindex_dev *d = p->index[P.col]->dev;
-          for (const PlannedBound *B : {&P.lo, &P.hi}) {
-            if (B->list < 0) continue;
+          if (P.narrow) {  // t_lo ^ t_hi = the bin's rows with lo <= x <= hi: one pass
             BatchList &L = g.list[g.nlists++];
-            L.dir = p->dir_base[P.col] + (size_t)B->list * per;
+            L.dir = p->dir_base[P.col] + (size_t)P.lo.list * per;
             L.bent = d->bent;
-            L.thr = B->thr;
+            L.thr = P.lo_v;
+            L.span = P.hi_v - P.lo_v;
             L.slot = (uint32_t)g.nslots;
+          } else {
+            for (const PlannedBound *B : {&P.lo, &P.hi}) {
+              if (B->list < 0) continue;
+              BatchList &L = g.list[g.nlists++];
+              L.dir = p->dir_base[P.col] + (size_t)B->list * per;
+              L.bent = d->bent;
+              L.thr = B->thr;
+              L.span = 0xFFFFFFFFu - B->thr;  // value >= thr
+              L.slot = (uint32_t)g.nslots;
+            }
           }
         }
         g.nslots++;

@@ -30,7 +30,7 @@ struct BoundDev {
   const BITS *fv;       // kBaseFv
   const uint32_t *dir;  // tile directory row of the bin (nullptr: no refine)
   const uint2 *bent;    // (row, value) entries
-  CODE thr;
+  CODE thr, span;       // list rows with thr <= x <= thr + span are marked
   uint32_t base;
 };
 
@@ -40,14 +40,15 @@ struct QueryArgs {
   BoundDev hi[C];  // L(b + 1)
 };
 
-// Rows of bound b's refine list in tile t with value >= thr: marked in the tile's LDS
-// toggles.  The list's tile slice [e0, e1) was read from the directory beforehand.
+// Rows of bound b's refine list in tile t with value in [thr, thr + span] (a bound: value
+// >= thr; a narrow column: its whole range): marked in the tile's LDS toggles.  The list's
+// tile slice [e0, e1) was read from the directory beforehand.
 __device__ __forceinline__ void mark_removed(const BoundDev &b, uint32_t e0, uint32_t e1, int64_t t,
                                              uint32_t *tog) {
   const uint32_t row0 = (uint32_t)(t * kTileRows);
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const uint2 pv = b.bent[e];
-    if (pv.y >= b.thr) atomicOr(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
+    if (pv.y - b.thr <= b.span) atomicOr(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
   }
 }
 
@@ -153,6 +154,7 @@ static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
     b.dir = d->bdir + (size_t)s.list * (d->ntiles + 1);
     b.bent = d->bent;
     b.thr = s.thr;
+    b.span = 0xFFFFFFFFu - s.thr;  // value >= thr
   }
   return b;
 }
@@ -187,6 +189,13 @@ static hipError_t query_group(const BinDex *const *bxs, const CODE *lo, const CO
     const BoundSpec sh = bound_spec(d, hi[c] + 1u, hi[c] == 0xFFFFFFFFu);
     q.lo[c] = make_bound(d, sl);
     q.hi[c] = make_bound(d, sh);
+    if (sl.list >= 0 && sl.list == sh.list) {
+      // both bounds in one bin: (1 ^ 0) & ~(1 ^ t) = t with t the bin's rows in [lo, hi],
+      // one pass over the list and no filter vector
+      q.lo[c].base = q.hi[c].base = kBaseOnes;
+      q.lo[c].span = hi[c] - lo[c];
+      q.hi[c].dir = nullptr;
+    }
   }
   if (mode == kStoreAnd) return launch_query<C, kStoreAnd>(q, bxs[0]->dev, out, count, s);
   return launch_query<C, kStoreSet>(q, bxs[0]->dev, out, count, s);

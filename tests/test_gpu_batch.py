@@ -68,6 +68,8 @@ def test_batch_matches_closed_form(eng, tile_words, qgroup, direct_share, order)
     plan.set_queries(queries)
     st = plan.stats()
     assert st["bindex_cols"] + st["direct_cols"] == sum(len(q) for q in queries)
+    if direct_share < 1.0:  # selective ranges put both bounds in one bin: one-pass lists
+        assert st["narrow_cols"] > 0
     stride = ((n + 31) // 32 + 3) // 4 * 4
     out = eng.DeviceBuffer(4 * stride * len(queries)).fill(0xEE)
     counts = eng.DeviceBuffer(8 * len(queries))

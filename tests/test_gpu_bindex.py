@@ -45,6 +45,11 @@ def query_bounds(x, sv, rng, count=40):
     for _ in range(8):
         a, b = sorted(int(v) for v in rng.integers(0, 2 ** 32, 2))
         out.append((a, b))
+    # both bounds inside one bin (a single pass over that bin's refine list)
+    for k in (0, 5, 64, 120, 126):
+        s0, s1 = int(sv[k]), int(sv[k + 1])
+        if s1 - s0 > 3:
+            out += [(s0 + 1, s1 - 1), (s0 + 1, s0 + 1), (s0 + 2, s1 - 2)]
     return out
 
 
