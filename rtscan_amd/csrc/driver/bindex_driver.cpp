@@ -2,7 +2,7 @@
 // line and the check output of the reference's GPU driver (bindex_cuda.cpp:1708-1995):
 //
 //   rts_bindex_driver [-f <data file>] [-b <columns>] [-p <query file>] [-n <rows>]
-//                     [-s <seed>] [-m single|batch]
+//                     [-s <seed>] [-m single|batch] [-e]
 //
 // Data: the data file (columns of n little-endian uint32, one after another), or seeded
 // uniform columns when no file is given (the reference draws unseeded random data).  Its
@@ -12,7 +12,11 @@
 // (rts_bindex_query), or -m batch runs all of them in one batched launch; its bitmap is
 // copied to page-locked host memory and checked against raw_scan of the same predicates
 // with compare_bitmap, which prints "[CHECK]hit true/total" (and "[ERROR] ..." at the first
-// wrong row).  Exit status: 0 when every query matched, 1 on a mismatch, 2 on an error.
+// wrong row).  -e encodes the columns first, as the reference's ENCODE build does
+// (rtscan.cpp:1068-1074, 1016-1026): normalEncode to rank codes in [0, n), every target
+// mapped through encodeQuery, the queries then run on the codes (eq and bt targets cannot
+// be encoded, as there).  Exit status: 0 when every query matched, 1 on a mismatch, 2 on
+// an error.
 #include <hip/hip_runtime_api.h>
 
 #include <stdio.h>
@@ -29,6 +33,7 @@
 #include "rts_bindex.h"
 #include "rts_check.h"
 #include "rts_query.h"
+#include "rts_remap.h"
 
 namespace {
 
@@ -50,15 +55,16 @@ const char *op_name(int op) {
 
 int main(int argc, char **argv) {
   std::string data_path, query_path = "test/scan_cmd.txt", mode = "single";
+  bool encode = false;
   int ncols = 1;
   long long n = 0;
   unsigned long long seed = 1;
   int opt;
-  while ((opt = getopt(argc, argv, "hf:b:p:n:s:m:")) != -1) {
+  while ((opt = getopt(argc, argv, "hf:b:p:n:s:m:e")) != -1) {
     switch (opt) {
       case 'h':
         printf("Usage: %s [-f <input-file>] [-b <columns>] [-p <scan-cmd-file>] [-n <rows>] [-s <seed>] "
-               "[-m single|batch]\n", argv[0]);
+               "[-m single|batch] [-e]\n", argv[0]);
         return 0;
       case 'f': data_path = optarg; break;
       case 'b': ncols = atoi(optarg); break;
@@ -66,6 +72,7 @@ int main(int argc, char **argv) {
       case 'n': n = atoll(optarg); break;
       case 's': seed = strtoull(optarg, nullptr, 10); break;
       case 'm': mode = optarg; break;
+      case 'e': encode = true; break;
       default:
         printf("Error: unknown option %c\n", (char)opt);
         return 2;
@@ -98,6 +105,20 @@ int main(int argc, char **argv) {
       if (rts_generate_column(0, seed, c, (uint64_t)n, 0.0, 0.0, cols[c], 0) != 0) return fail("generate");
   }
 
+  if (encode) {  // rank codes in [0, n), in place (rtscan.cpp:1072)
+    std::vector<POSTYPE *> sorted_pos(ncols, nullptr);
+    std::vector<CODE *> sorted_data(ncols, nullptr);
+    auto te = std::chrono::steady_clock::now();
+    if (!normalEncode(cols.data(), ncols, 0, (CODE)n, (int)n, sorted_pos.data(), sorted_data.data()))
+      return fail("normalEncode");
+    printf("[Time] Uniform Encoding: %.3f ms\n", ms_since(te));
+    for (int c = 0; c < ncols; c++) {
+      printf("[+] column %d: %zu distinct values\n", c, encodeMap[c].size());
+      free(sorted_pos[c]);
+      free(sorted_data[c]);
+    }
+  }
+
   // indexes (bindex_cuda.cpp:1763-1780)
   std::vector<BinDex> bx(ncols);
   size_t free0 = 0, total = 0, free1 = 0, base_data = 0;
@@ -122,6 +143,24 @@ int main(int argc, char **argv) {
   const int nq = rts_query_file_count(qf);
   printf("BinDex scan... (%d queries, %s)\n", nq, mode.c_str());
 
+  // the targets of (query, column), mapped through encodeQuery with -e (rtscan.cpp:1022)
+  auto target = [&](int q, int c, int which) -> CODE {
+    const rts_column_predicate *p = rts_query_file_get(qf, q, c);
+    const CODE t = which == 1 ? p->t1[0] : (p->t2 ? p->t2[0] : 0);
+    if (!encode || which != 1) return t;
+    std::string cmd = op_name(p->op);
+    return encodeQuery(c, t, cmd);
+  };
+  if (encode)
+    for (int q = 0; q < nq; q++)
+      for (int c = 0; c < ncols; c++) {
+        const int op = rts_query_file_get(qf, q, c)->op;
+        if (op == EQ || op == BT) {
+          fprintf(stderr, "[ERROR] query %d column %d: eq and bt targets cannot be encoded\n", q, c);
+          return 2;
+        }
+      }
+
   // closed ranges of every query
   std::vector<rts_query_spec> specs(nq);
   for (int q = 0; q < nq; q++) {
@@ -135,7 +174,7 @@ int main(int argc, char **argv) {
         return 2;
       }
       sp.column[c] = c;
-      if (!rts_predicate_interval((enum OPERATOR)p->op, p->t1[0], p->t2 ? p->t2[0] : 0, &sp.lo[c], &sp.hi[c])) {
+      if (!rts_predicate_interval((enum OPERATOR)p->op, target(q, c, 1), target(q, c, 2), &sp.lo[c], &sp.hi[c])) {
         sp.lo[c] = 1;  // empty: lo > hi
         sp.hi[c] = 0;
       }
@@ -181,6 +220,8 @@ int main(int argc, char **argv) {
       const rts_column_predicate *p = rts_query_file_get(qf, q, c);
       if (p->op == BT)
         printf("%s %u %u\n", op_name(p->op), p->t1[0], p->t2[0]);
+      else if (encode)
+        printf("%s %u [ENCODE] %u\n", op_name(p->op), p->t1[0], target(q, c, 1));
       else
         printf("%s %u\n", op_name(p->op), p->t1[0]);
     }
@@ -209,7 +250,7 @@ int main(int argc, char **argv) {
     printf("[CHECK]check final result.\n");
     for (int c = 0; c < ncols; c++) {
       const rts_column_predicate *p = rts_query_file_get(qf, q, c);
-      CODE a = p->t1[0], b = p->t2 ? p->t2[0] : 0;
+      CODE a = target(q, c, 1), b = target(q, c, 2);
       if (p->op == BT && a > b) std::swap(a, b);
       std::fill(colbm.begin(), colbm.end(), 0u);
       raw_scan(&bx[c], colbm.data(), a, b, (enum OPERATOR)p->op, cols[c], nullptr);

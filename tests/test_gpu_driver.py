@@ -69,3 +69,45 @@ def test_data_file_and_counts(eng, tmp_path):
         assert "N = %d" % n in out
         h = hits(out)
         assert [b for _, b in h] == want and all(a == b for a, b in h)
+
+
+def test_encode_mode_preserves_lt_answers(eng):
+    """-e rank-encodes the columns with normalEncode and maps targets with encodeQuery, as
+    the reference's ENCODE build does (rtscan.cpp:1016-1026, 1068-1074): every query still
+    matches raw_scan on the codes, and because rank codes keep the value order, each `lt`
+    query selects as many rows as on the values."""
+    q = os.path.join(GOLDEN, "scan_cmd_32_3c.txt")
+    args = ["-b", "3", "-p", q, "-n", "500009", "-s", "11"]
+    rc0, out0, err0 = run(args)
+    rc1, out1, err1 = run(args + ["-e"])
+    assert rc0 == 0 and rc1 == 0, (err0[-1000:], err1[-1000:])
+    assert "[ENCODE]" in out1 and "distinct values" in out1
+    h0, h1 = hits(out0), hits(out1)
+    assert all(a == b for a, b in h1)
+    assert [b for _, b in h1] == [b for _, b in h0]
+
+
+def test_encode_mode_targets(eng, tmp_path):
+    """encodeQuery's operator rules on real data: le and ge keep the value answers; gt
+    takes the start of the next value, so on the codes it misses that value's first row
+    (the reference's rule, remap.cpp:256-261); eq / bt cannot be encoded (exit 2)."""
+    n = 200_003
+    x = R.generate_column(R.DIST_ZIPF, 9, 0, n, 1.5, 2.0 ** 10)
+    path = str(tmp_path / "d.bin")
+    x.astype("<u4").tofile(path)
+    vals = np.unique(x)
+    qs = [int(vals[3]), int(vals[len(vals) // 2]), int(vals[-2])]
+    lines = []
+    for v in qs:
+        lines += ["le %d" % v, "ge %d" % v, "gt %d" % v]
+    (tmp_path / "q.txt").write_text("\n".join(lines + ["exit"]) + "\n")
+    rc, out, err = run(["-f", path, "-b", "1", "-p", str(tmp_path / "q.txt"), "-e"])
+    assert rc == 0, err[-1000:]
+    h = hits(out)
+    want = []
+    for v in qs:
+        want += [int((x <= v).sum()), int((x >= v).sum()), int((x > v).sum()) - 1]
+    assert [b for _, b in h] == want and all(a == b for a, b in h)
+    (tmp_path / "e.txt").write_text("eq 5\nexit\n")
+    rc, out, err = run(["-f", path, "-b", "1", "-p", str(tmp_path / "e.txt"), "-e"])
+    assert rc == 2 and "cannot be encoded" in err
