@@ -137,3 +137,42 @@ def test_query_driver_is_built_and_prints_usage():
     assert p.returncode == 0 and "-p <scan-cmd-file>" in p.stdout
     p = subprocess.run([drv, "-b", "9", "-n", "1000"], capture_output=True, text=True, timeout=30)
     assert p.returncode == 2
+
+
+HIP_ERROR_INVALID_VALUE = 1
+
+
+def test_entry_points_reject_bad_arguments_before_any_device_work():
+    """Argument errors come back as hipErrorInvalidValue with a message, before the entry
+    point touches the device (so this runs without a GPU)."""
+    L = R.lib()
+    # snapshot that does not exist
+    from rtscan_amd.bindex_struct import BinDex
+    bx = BinDex()
+    assert L.rts_bindex_load(ctypes.byref(bx), b"/nonexistent/dir/x.rtsbdx", None) == HIP_ERROR_INVALID_VALUE
+    assert b"rts_bindex_load" in L.rts_last_error()
+    # batch plan: bad shape, no columns
+    cfg = R.BatchConfig()
+    L.rts_batch_default_config(ctypes.byref(cfg))
+    plan = ctypes.c_void_p()
+    cols = (ctypes.c_void_p * 1)(None)
+    for bad in ((300, 16, 2, 1), (1024, 0, 2, 1), (1024, 16, 0, 1), (1024, 16, 2, 7)):
+        c = R.BatchConfig(*bad)
+        assert L.rts_batch_plan_create(ctypes.byref(plan), cols, None, 1, 1000, 4, ctypes.byref(c),
+                                       None) == HIP_ERROR_INVALID_VALUE, bad
+    assert L.rts_batch_plan_create(ctypes.byref(plan), cols, None, 0, 1000, 4, ctypes.byref(cfg),
+                                   None) == HIP_ERROR_INVALID_VALUE
+    # server: zero batch size / streams
+    sc = R.ServerConfig()
+    L.rts_server_default_config(ctypes.byref(sc))
+    sc.max_batch = 0
+    srv = ctypes.c_void_p()
+    assert L.rts_server_create(ctypes.byref(srv), ctypes.byref(sc)) == HIP_ERROR_INVALID_VALUE
+    # checker and encoders: missing pointers / sizes
+    assert L.rts_bitmap_compare(None, None, 10, None, None) == HIP_ERROR_INVALID_VALUE
+    d = ctypes.c_ulonglong()
+    assert L.rts_bitmap_digest(None, 10, ctypes.byref(d), None) == HIP_ERROR_INVALID_VALUE
+    m = ctypes.c_int64()
+    assert L.rts_encode_column(None, 10, 0, 10, None, None, None, None, None, ctypes.byref(m),
+                               None) == HIP_ERROR_INVALID_VALUE
+    assert L.rts_scan_range(None, 10, 0, 5, None, None, None) == HIP_ERROR_INVALID_VALUE
