@@ -46,6 +46,8 @@ typedef struct rts_batch_config {
   int queries_per_group;
   int blocks_per_cu;
   int order;
+  int threads;  /* workgroup size: 512 (64 KB of LDS toggles) or 256 (32 KB, half the
+                   column slots per work item, twice the work items per CU) */
 } rts_batch_config;
 
 typedef struct rts_batch_plan rts_batch_plan;

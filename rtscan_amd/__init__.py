@@ -545,13 +545,15 @@ class QuerySpec(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("tile_words", ctypes.c_int), ("queries_per_group", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("order", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("order", ctypes.c_int), ("threads", ctypes.c_int)]
 
     def astuple(self):
-        return (self.tile_words, self.queries_per_group, self.blocks_per_cu, self.order)
+        return (self.tile_words, self.queries_per_group, self.blocks_per_cu, self.order,
+                self.threads)
 
 
-def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, order=None):
+def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, order=None,
+                 threads=None):
     c = BatchConfig()
     lib().rts_batch_default_config(ctypes.byref(c))
     if tile_words is not None:
@@ -562,6 +564,8 @@ def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, or
         c.blocks_per_cu = blocks_per_cu
     if order is not None:
         c.order = order
+    if threads is not None:
+        c.threads = threads
     return c
 
 

@@ -41,7 +41,8 @@ bool cache_lookup(const char *path, const std::string &key, rts_batch_config *cf
     std::string k;
     rts_batch_config c;
     float m;
-    if (ls >> k >> c.tile_words >> c.queries_per_group >> c.blocks_per_cu >> c.order >> m && k == key) {
+    if (ls >> k >> c.tile_words >> c.queries_per_group >> c.blocks_per_cu >> c.order >> c.threads >> m &&
+        k == key) {
       *cfg = c;  // the last entry for a key wins
       *ms = m;
       found = true;
@@ -53,8 +54,8 @@ bool cache_lookup(const char *path, const std::string &key, rts_batch_config *cf
 void cache_store(const char *path, const std::string &key, const rts_batch_config &c, float ms) {
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %.4f\n", key.c_str(), c.tile_words, c.queries_per_group, c.blocks_per_cu,
-          c.order, ms);
+  fprintf(f, "%s %d %d %d %d %d %.4f\n", key.c_str(), c.tile_words, c.queries_per_group, c.blocks_per_cu,
+          c.order, c.threads, ms);
   fclose(f);
 }
 

@@ -33,8 +33,14 @@ namespace rts {
 
 enum : uint32_t { kBaseZero = 0, kBaseOnes = 1, kBaseFv = 2 };
 
-constexpr int kBatchThreads = 512;  // 8 waves per workgroup, 2 workgroups per CU (LDS)
-constexpr int kBatchWaves = kBatchThreads / kWave;
+// Workgroups of 512 threads (8 waves, 64 KB of toggles, 2 per CU) or 256 threads (4 waves,
+// 32 KB, up to 4 per CU: more independent work items in flight per CU).
+template <int NT>
+struct BatchShape {
+  static constexpr int kThreads = NT;
+  static constexpr int kWaves = NT / kWave;
+  static constexpr int kLdsWords = kBatchLdsWords / 512 * NT;  // toggle words per workgroup
+};
 
 // One query-column of a group.  BinDex form: (fv_hi & ~fv_lo) ^ T, with T the toggles of
 // the slot's refine lists; bases kBaseZero / kBaseOnes need no filter vector.  Direct form
@@ -79,9 +85,9 @@ struct BatchGroup {
 };
 
 // LDS of one workgroup: one toggle array per slot and the current item's descriptors.
-template <int TW>
+template <int TW, int NT>
 struct BatchLds {
-  static constexpr int kSlots = kBatchLdsWords / TW;
+  static constexpr int kSlots = BatchShape<NT>::kLdsWords / TW;
   static constexpr int kLists = 2 * kSlots;
   uint32_t tog[kSlots][TW];
   const uint2 *lent[kLists];  // first entry of the list's tile slice
@@ -100,11 +106,11 @@ struct BatchLds {
 // Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW / 8
 // words, 256 rows (8 words) per step, 4 steps' loads in flight, with the DPP assembly of
 // k_scan_tiles.  Rows past n give 0 bits.
-template <int TW>
+template <int TW, int NT>
 __device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int64_t n, uint32_t *dst) {
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   const uint32_t shift = 28 - 4 * (lane & 7);
-  constexpr int kWordsPerWave = TW / kBatchWaves;
+  constexpr int kWordsPerWave = TW / BatchShape<NT>::kWaves;
   constexpr int kSteps = kWordsPerWave / 8;
   constexpr int kU = 4;
   static_assert(kSteps % kU == 0, "steps per wave");
@@ -197,13 +203,14 @@ __device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
   }
 }
 
-template <int TW>
-__global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
+template <int TW, int NT>
+__global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
     const BatchGroup *__restrict__ groups, const int *__restrict__ ngroups_ptr, int64_t n,
     int64_t words, int64_t ntiles, BITS *out, int64_t out_stride, uint32_t *partials) {
+  constexpr int kBatchThreads = NT, kBatchWaves = BatchShape<NT>::kWaves;
   constexpr int kPer = TW / kBatchThreads;  // output words per thread
-  static_assert(kPer == 1 || kPer == 2 || kPer == 4, "tile words per thread");
-  using Lds = BatchLds<TW>;
+  static_assert(kPer == 1 || kPer == 2 || kPer == 4 || kPer == 8, "tile words per thread");
+  using Lds = BatchLds<TW, NT>;
   constexpr int kSlots = Lds::kSlots, kLists = Lds::kLists;
   static_assert(kLists <= kWave && kSlots <= kWave, "descriptor loads by one wave each");
   __shared__ Lds lds;
@@ -242,9 +249,8 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
       lds.qout[qi] = Q.out ? Q.out : out + (int64_t)Q.id * out_stride;
       lds.qempty[qi] = Q.empty;
       lds.qid[qi] = (uint32_t)Q.id;
-    } else if (tid >= 4 * kWave && tid <= 4 * kWave + kBatchMaxGroup) {
-      lds.qslot[tid - 4 * kWave] = g->qslot[tid - 4 * kWave];
     }
+    if (tid <= kBatchMaxGroup) lds.qslot[tid] = g->qslot[tid];
     const int nslots = g->nslots, nq = g->nq;
     __syncthreads();
 
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(kBatchThreads) __attribute__((amdgpu_waves_per_eu(4
       }
     }
     for (int s = 0; s < nslots; s++)
-      if (lds.slot[s].col) batch_direct<TW>(lds.slot[s], t, n, lds.tog[s]);
+      if (lds.slot[s].col) batch_direct<TW, NT>(lds.slot[s], t, n, lds.tog[s]);
     __syncthreads();
 
     // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
@@ -480,6 +486,7 @@ __global__ void k_retile_directory(const uint2 *bent, const uint32_t *boff, int6
 
 static bool valid_config(const rts_batch_config &c) {
   return (c.tile_words == 512 || c.tile_words == 1024 || c.tile_words == 2048) && (c.order == 0 || c.order == 1) &&
+         (c.threads == 256 || c.threads == 512) &&
          c.queries_per_group >= 1 &&
          c.queries_per_group <= kBatchMaxGroup && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16;
 }
@@ -522,40 +529,41 @@ static hipError_t build_dirs(rts_batch_plan *p, hipStream_t s) {
 namespace rts {
 
 // Workgroups of k_batch<TW> that fit on one compute unit at once (LDS-bound).
-template <int TW>
+template <int TW, int NT>
 static int batch_resident_per_cu() {
   static int cached = 0;
   if (!cached) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<TW>, kBatchThreads, 0) != hipSuccess || nb < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_batch<TW, NT>, NT, 0) != hipSuccess || nb < 1)
       nb = 1;
     cached = nb;
   }
   return cached;
 }
 
-static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
-                               unsigned long long *counts, hipStream_t s) {
+template <int TW, int NT>
+static void launch_k_batch(rts_batch_plan *p, BITS *out, int64_t out_stride, hipStream_t s) {
   const int cus = std::max(1, device_cu_count());
   // a persistent grid larger than what is resident would leave its statically assigned
   // work items to a second round: clamp to the occupancy limit
-  const int resident = p->cfg.tile_words == 2048   ? batch_resident_per_cu<2048>()
-                       : p->cfg.tile_words == 1024 ? batch_resident_per_cu<1024>()
-                                                   : batch_resident_per_cu<512>();
-  unsigned grid = (unsigned)(cus * std::min(p->cfg.blocks_per_cu, resident));
+  unsigned grid = (unsigned)(cus * std::min(p->cfg.blocks_per_cu, batch_resident_per_cu<TW, NT>()));
   grid = std::max<unsigned>(kXcds, grid / kXcds * kXcds);
+  hipLaunchKernelGGL((k_batch<TW, NT>), dim3(grid), dim3(NT), 0, s, p->d_groups, p->d_ngroups, p->n,
+                     p->words, p->ntiles, out, out_stride, p->d_partials);
+}
+
+static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
+                               unsigned long long *counts, hipStream_t s) {
+  const bool small = p->cfg.threads == 256;
   switch (p->cfg.tile_words) {
     case 2048:
-      hipLaunchKernelGGL(k_batch<2048>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
-                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      small ? launch_k_batch<2048, 256>(p, out, out_stride, s) : launch_k_batch<2048, 512>(p, out, out_stride, s);
       break;
     case 1024:
-      hipLaunchKernelGGL(k_batch<1024>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
-                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      small ? launch_k_batch<1024, 256>(p, out, out_stride, s) : launch_k_batch<1024, 512>(p, out, out_stride, s);
       break;
     default:
-      hipLaunchKernelGGL(k_batch<512>, dim3(grid), dim3(kBatchThreads), 0, s, p->d_groups, p->d_ngroups,
-                         p->n, p->words, p->ntiles, out, out_stride, p->d_partials);
+      small ? launch_k_batch<512, 256>(p, out, out_stride, s) : launch_k_batch<512, 512>(p, out, out_stride, s);
   }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess || !counts) return e;
@@ -573,6 +581,7 @@ void rts_batch_default_config(rts_batch_config *cfg) {
   cfg->queries_per_group = 16;
   cfg->blocks_per_cu = 4;
   cfg->order = 1;
+  cfg->threads = 512;
 }
 
 hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *columns,
@@ -619,7 +628,7 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *
                                      hipStream_t stream) {
   if (!p || !cfg || !valid_config(*cfg)) return set_error(hipErrorInvalidValue, "rts_batch_plan_set_config");
   const bool retile = cfg->tile_words != p->cfg.tile_words;
-  const bool reorder = cfg->order != p->cfg.order;
+  const bool reorder = cfg->order != p->cfg.order || cfg->threads != p->cfg.threads;
   p->cfg = *cfg;
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
@@ -724,7 +733,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   }
   // 3. groups: consecutive queries (in that order) while their slots fit the LDS of one
   //    tile and the configured group size, each group one self-contained record
-  const int slot_cap = kBatchLdsWords / p->cfg.tile_words;
+  const int slot_cap = kBatchLdsWords / 512 * p->cfg.threads / p->cfg.tile_words;
   const int64_t per = p->ntiles + 1;
   std::vector<BatchGroup> groups;
   for (int oi = 0; oi < nq;) {

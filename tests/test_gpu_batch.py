@@ -55,15 +55,16 @@ def check_results(eng, cols, queries, out, counts, stride):
         assert int(cnt[i]) == int(m.sum()), (i, q)
 
 
-@pytest.mark.parametrize("tile_words,qgroup,direct_share,order", [
-    (1024, 16, 0.0, 1), (1024, 16, 0.0, 0), (1024, 4, 0.3, 1), (2048, 8, 0.0, 1),
-    (2048, 64, 1.0, 0), (1024, 1, 0.5, 1), (512, 32, 0.0, 1), (512, 64, 0.4, 0), (512, 3, 1.0, 1)])
-def test_batch_matches_closed_form(eng, tile_words, qgroup, direct_share, order):
+@pytest.mark.parametrize("tile_words,qgroup,direct_share,order,threads", [
+    (1024, 16, 0.0, 1, 512), (1024, 16, 0.0, 0, 512), (1024, 4, 0.3, 1, 512), (2048, 8, 0.0, 1, 512),
+    (2048, 64, 1.0, 0, 512), (1024, 1, 0.5, 1, 512), (512, 32, 0.0, 1, 512), (512, 64, 0.4, 0, 512),
+    (512, 3, 1.0, 1, 512), (2048, 16, 0.1, 1, 256), (1024, 64, 0.0, 0, 256), (512, 8, 1.0, 1, 256)])
+def test_batch_matches_closed_form(eng, tile_words, qgroup, direct_share, order, threads):
     n = (1 << 20) + 12345  # partial last tile, partial last word
     cols, dcols, idx, sc = make_table(eng, n)
     rng = np.random.default_rng(tile_words + qgroup)
     queries = random_queries(rng, sc, 150, direct_share)
-    cfg = eng.batch_config(tile_words, qgroup, 4, order)
+    cfg = eng.batch_config(tile_words, qgroup, 4, order, threads)
     plan = eng.BatchPlan(dcols, idx, n, len(queries), cfg)
     plan.set_queries(queries)
     st = plan.stats()
