@@ -8,9 +8,10 @@
  * BinDex when the column has one.  Query q's bitmap is written at out + q * out_stride_words
  * (MSB-first words, bits past row n cleared) and its popcount at counts[q] (device).
  *
- * The launch is one persistent kernel over (row tile, query group) work items, assigned so
- * that one tile's work items run on one XCD back to back, plus one small kernel summing the
- * per-(query, tile) popcounts.  rts_batch_plan_graph_launch captures both in a hipGraph and
+ * The launch is one persistent kernel over (row tile, query group) work items, handed out by
+ * per-XCD counters so that one tile's work items run on one XCD back to back, plus one small
+ * kernel summing the per-(query, tile) popcounts.  Launches of one plan must not overlap
+ * (use one stream per plan, or order them): they share the plan's counters and partials.  rts_batch_plan_graph_launch captures both in a hipGraph and
  * replays it; queries set with rts_batch_plan_set_queries live in device memory at a fixed
  * address, so a replay evaluates the current set.
  */
