@@ -370,7 +370,7 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
     if config:  # fixed launch shape (profiling runs)
         cands = [R.batch_config(*config)]
     else:
-        cands = [R.batch_config(t, g, 4, o, nt) for t in (512, 1024, 2048) for g in (16, 64)
+        cands = [R.batch_config(t, g, 4, o, nt) for t in (512, 1024, 2048) for g in (16, 32, 64)
                  for o in (0, 1) for nt in (256, 512)]
     t0 = time.perf_counter()
     chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,

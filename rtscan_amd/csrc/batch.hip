@@ -106,6 +106,18 @@ struct BatchLds {
   unsigned long long next;  // the workgroup's next work item
 };
 
+constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
+
+// Toggle layout: thread t owns tile words t * kPer .. t * kPer + kPer - 1 (kPer = TW / NT),
+// stored transposed, word w of the tile at (w % kPer) * NT + w / kPer, so that the
+// accesses of each thread's k-th word (filter-vector XORs, the AND phase, the clearing)
+// are consecutive across lanes and free of bank conflicts.
+template <int TW, int NT>
+__device__ __forceinline__ uint32_t tog_index(uint32_t w) {
+  constexpr int kPer = TW / NT;
+  return ((w & (kPer - 1)) << ilog2(NT)) | (w >> ilog2(kPer));
+}
+
 // Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW / 8
 // words, 256 rows (8 words) per step, 4 steps' loads in flight, with the DPP assembly of
 // k_scan_tiles.  Rows past n give 0 bits.
@@ -129,7 +141,7 @@ __device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int6
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         const uint32_t w = or_lanes8(nibble4(v[u], c.dlo, c.dspan) << shift);
-        if ((lane & 7) == 0) dst[wv * kWordsPerWave + (s0 + u) * 8 + lane / 8] = w;
+        if ((lane & 7) == 0) dst[tog_index<TW, NT>(wv * kWordsPerWave + (s0 + u) * 8 + lane / 8)] = w;
       }
     }
     return;
@@ -144,7 +156,7 @@ __device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int6
       if (r < n && (c.col[r] - c.dlo) <= c.dspan) nib |= 8u >> j;
     }
     const uint32_t w = or_lanes8(nib << shift);
-    if ((lane & 7) == 0) dst[wv * kWordsPerWave + s * 8 + lane / 8] = w;
+    if ((lane & 7) == 0) dst[tog_index<TW, NT>(wv * kWordsPerWave + s * 8 + lane / 8)] = w;
   }
 }
 
@@ -318,7 +330,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (s0 + u < nslots && !S.col && w < words) {
 #pragma unroll
               for (int k = 0; k < kPer; k++)
-                atomicXor(&lds.tog[s0 + u][lw + k], h[u].v[k] & ~l[u].v[k]);
+                atomicXor(&lds.tog[s0 + u][k * kBatchThreads + tid], h[u].v[k] & ~l[u].v[k]);
             }
           }
         }
@@ -333,7 +345,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
           const uint32_t e = off + u * kWave + lane;
           // rows of the slice lie in [row0, row0 + TW * 32): the word index is < TW
           if (e < llen[i] && ent[i][u].y - thr <= span)
-            atomicXor(&tg[(ent[i][u].x - row0) >> 5], 0x80000000u >> (ent[i][u].x & 31));
+            atomicXor(&tg[tog_index<TW, NT>((ent[i][u].x - row0) >> 5)], 0x80000000u >> (ent[i][u].x & 31));
         }
       }
     }
@@ -350,7 +362,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       for (int k = 0; k < kPer; k++) acc[k] = fill;
       for (uint32_t s = lds.qslot[qi]; s < lds.qslot[qi + 1]; s++) {
 #pragma unroll
-        for (int k = 0; k < kPer; k++) acc[k] &= lds.tog[s][lw + k];
+        for (int k = 0; k < kPer; k++) acc[k] &= lds.tog[s][k * kBatchThreads + tid];
       }
       uint32_t pc = 0;
       if (w < words) {
@@ -374,7 +386,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     for (int s = 0; s < nslots; s++)
 #pragma unroll
-      for (int k = 0; k < kPer; k++) lds.tog[s][lw + k] = 0;
+      for (int k = 0; k < kPer; k++) lds.tog[s][k * kBatchThreads + tid] = 0;
     // the next item's descriptors overwrite lds.qid as soon as its loads return: read it
     // before the barrier
     const uint32_t qid = tid < nq ? lds.qid[tid] : 0;
