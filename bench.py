@@ -458,7 +458,7 @@ def serve_queries(rng, table_cols, nq, ncols_max):
 
 
 def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000_000_000,
-                seed=55, pool_gb=32):
+                seed=55, pool_gb=32, batch_config=None):
     """Config 5: two resident tables, T1 = 2^26 rows x 2 uniform columns, T2 = n2 rows x 4
     columns (uniform, normal, Zipf 1.1, Zipf 1.5), a BinDex per column; Poisson arrivals at
     each rate, every query on T1 or T2 with probability 1/2, shapes as config 4; admission by
@@ -479,11 +479,14 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
     t1_samples = [np.sort(c[rng.integers(0, n1, 1_000_000)]) for c in t1_cols]
     setup_s = time.time() - t_setup
     cfg = R.server_config(max_batch=256, streams=3, pool_bytes=pool_gb << 30)
+    if batch_config is not None:
+        cfg.batch = R.batch_config(*batch_config)
     srv = R.Server(cfg)
     tid1 = srv.add_table(t1_d, t1_ix, n1)
     tid2 = srv.add_table(t2_d, t2_ix, n2)
     srv.start()
-    out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb, "rates": []}
+    out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb,
+           "batch_config": list(cfg.batch.astuple()), "rates": []}
     for rate, secs in zip(rates, seconds):
         nq = int(rate * secs)
         tids = [tid1 if rng.random() < 0.5 else tid2 for _ in range(nq)]
@@ -542,8 +545,8 @@ def main():
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
-                    help="tile_words,queries_per_group,blocks_per_cu[,order] for the batch "
-                         "workload (skips the autotuner)")
+                    help="tile_words,queries_per_group,blocks_per_cu[,order[,threads]] for the "
+                         "batch workload (skips the autotuner) or the serve workload's batches")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction, batched and serving legs of the default run")
     args = ap.parse_args()
@@ -560,7 +563,8 @@ def main():
     if args.workload == "copycal":
         res = bench_copycal(args, R)
     elif args.workload == "serve":
-        res = {"serve": bench_serve(R, n2=args.serve_rows)}
+        cfg = tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config else None
+        res = {"serve": bench_serve(R, n2=args.serve_rows, batch_config=cfg)}
     elif args.workload in ("conj", "batch"):
         table = Table(R)
         res = {"table_build_ms": table.build_ms}

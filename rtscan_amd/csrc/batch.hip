@@ -611,8 +611,8 @@ static hipError_t launch_batch(rts_batch_plan *p, BITS *out, int64_t out_stride,
 extern "C" {
 
 void rts_batch_default_config(rts_batch_config *cfg) {
-  cfg->tile_words = 1024;
-  cfg->queries_per_group = 16;
+  cfg->tile_words = 2048;
+  cfg->queries_per_group = 32;
   cfg->blocks_per_cu = 4;
   cfg->order = 1;
   cfg->threads = 256;
