@@ -86,18 +86,12 @@ struct BatchGroup {
   BatchQuery query[kBatchMaxGroup];
 };
 
-// LDS of one workgroup: one toggle array per slot and the current item's descriptors.
+// LDS of one workgroup: one toggle array per slot and the current item's query descriptors.
 template <int TW, int NT>
 struct BatchLds {
   static constexpr int kSlots = BatchShape<NT>::kLdsWords / TW;
   static constexpr int kLists = 2 * kSlots;
-  uint32_t tog[kSlots][TW];
-  const uint2 *lent[kLists];  // first entry of the list's tile slice
-  uint32_t llen[kLists];      // entries in the slice
-  CODE lthr[kLists];
-  CODE lspan[kLists];
-  uint32_t lslot[kLists];
-  BatchSlot slot[kSlots];
+  alignas(16) uint32_t tog[kSlots][TW];
   BITS *qout[kBatchMaxGroup];
   uint32_t qempty[kBatchMaxGroup];
   uint32_t qid[kBatchMaxGroup];
@@ -108,14 +102,24 @@ struct BatchLds {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
-// Toggle layout: thread t owns tile words t * kPer .. t * kPer + kPer - 1 (kPer = TW / NT),
-// stored transposed, word w of the tile at (w % kPer) * NT + w / kPer, so that the
-// accesses of each thread's k-th word (filter-vector XORs, the AND phase, the clearing)
-// are consecutive across lanes and free of bank conflicts.
+// Toggle layout: thread t owns tile words t * kPer .. t * kPer + kPer - 1 (kPer = TW / NT)
+// as kPer / U units of U words (U = 2 when kPer >= 2: 64-bit LDS accesses), stored
+// transposed: unit k of thread t at unit position k * NT + t.  The accesses of each
+// thread's k-th unit (filter-vector XORs, the AND phase, the clearing) are consecutive
+// across lanes and free of bank conflicts, with half the instructions of 32-bit words.
+template <int TW, int NT>
+struct TogLayout {
+  static constexpr int kPer = TW / NT;
+  static constexpr int kUnit = kPer >= 2 ? 2 : 1;  // words per unit
+  static constexpr int kUnits = kPer / kUnit;      // units per thread
+};
+
 template <int TW, int NT>
 __device__ __forceinline__ uint32_t tog_index(uint32_t w) {
-  constexpr int kPer = TW / NT;
-  return ((w & (kPer - 1)) << ilog2(NT)) | (w >> ilog2(kPer));
+  using T = TogLayout<TW, NT>;
+  const uint32_t u = w >> ilog2(T::kUnit), r = w & (T::kUnit - 1);
+  const uint32_t pos = ((u & (T::kUnits - 1)) << ilog2(NT)) | (u >> ilog2(T::kUnits));
+  return (pos << ilog2(T::kUnit)) | r;
 }
 
 // Direct form: the predicate words of this tile into `dst` (LDS).  Each wave covers TW / 8
@@ -199,6 +203,17 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
 
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
+// A value that is the same in every lane, moved to a scalar register (the builtin takes and
+// returns 32-bit values: a pointer goes one half at a time).
+__device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+  const uint64_t a = (uint64_t)p;
+  return (T *)(((uint64_t)uniform_u32((uint32_t)(a >> 32)) << 32) | uniform_u32((uint32_t)a));
+}
+
 // Result words are written once and not read again by the launch: non-temporal stores keep
 // them from evicting the filter-vector and list slices that the tile's other query groups
 // re-read from L2.
@@ -226,6 +241,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   constexpr int kBatchThreads = NT, kBatchWaves = BatchShape<NT>::kWaves;
   constexpr int kPer = TW / kBatchThreads;  // output words per thread
   static_assert(kPer == 1 || kPer == 2 || kPer == 4 || kPer == 8, "tile words per thread");
+  using TL = TogLayout<TW, NT>;
+
   using Lds = BatchLds<TW, NT>;
   constexpr int kSlots = Lds::kSlots, kLists = Lds::kLists;
   static_assert(kLists <= kWave && kSlots <= kWave, "descriptor loads by one wave each");
@@ -251,55 +268,46 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const BatchGroup *g = groups + it % ngroups;
     const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
 
-    // 1. the item's descriptors into LDS: every load of this round is independent of the
-    //    others; the list's tile slice (two directory words) is the only second round
-    if (tid < kLists) {
-      const BatchList L = g->list[tid];
-      const uint32_t st = L.dir[t];
-      lds.lent[tid] = L.bent + st;
-      lds.llen[tid] = L.dir[t + 1] - st;
-      lds.lthr[tid] = L.thr;
-      lds.lspan[tid] = L.span;
-      lds.lslot[tid] = L.slot;
-    } else if (tid >= kWave && tid < kWave + kSlots) {
-      lds.slot[tid - kWave] = g->slot[tid - kWave];
-    } else if (tid >= 2 * kWave && tid < 2 * kWave + kBatchMaxGroup) {
-      const int qi = tid - 2 * kWave;
-      const BatchQuery Q = g->query[qi];
-      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)Q.id * out_stride;
-      lds.qempty[qi] = Q.empty;
-      lds.qid[qi] = (uint32_t)Q.id;
-    }
-    if (tid <= kBatchMaxGroup) lds.qslot[tid] = g->qslot[tid];
+    // 1. Descriptors.  Each wave reads the records of its own refine lists and the slots
+    //    itself (wave-uniform addresses, no exchange through LDS, so no barrier and no wait
+    //    on another wave's directory loads); the query descriptors, needed only by the AND
+    //    phase, go to LDS before this item's barrier.  Only the used slot and query entries
+    //    are read, so a tile's group records take less L2.
     const int nslots = g->nslots, nq = g->nq;
     unsigned long long nxt = 0;
     if (tid == 0) nxt = atomicAdd(next_item, 1ull);  // returns while this item runs
-    __syncthreads();
+    const int qi = tid - 2 * kWave;
+    const bool qload = tid >= 2 * kWave && qi < nq;
+    BatchQuery Q{};
+    if (qload) Q = g->query[qi];
+    uint32_t qs = 0;
+    if (tid <= nq) qs = g->qslot[tid];
 
     const int64_t w = t * TW + lw;
+    const bool interior = (t + 1) * TW < words;  // the last bitmap word lies in a later tile
     // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time
-    //    (16 at 512-word tiles; their pointers come from the group record through scalar
-    //    loads), XORed into the toggles as they arrive: every toggle update is an XOR, so
-    //    the order does not matter and the words need not stay in registers.  Refine
-    //    entries: wave v owns lists v, v + 8, ...; a list's slice base is uniform across the
-    //    wave, each lane takes kListUnroll entries of each of its lists per round, and the
-    //    first round's loads go out together with the filter-vector loads.
+    //    (16 at 512-word tiles), XORed into the toggles as they arrive: every toggle update
+    //    is an XOR, so the order does not matter and the words need not stay in registers.
+    //    Refine entries: wave v owns lists v, v + 8, ...; a list's slice base is uniform
+    //    across the wave, each lane takes kListUnroll entries of each of its lists per
+    //    round, and the first round's loads go out together with the filter-vector loads.
     constexpr int kListsPerWave = kLists / kBatchWaves;
     constexpr int kListUnroll = 16 / kListsPerWave;
     static_assert(kLists % kBatchWaves == 0, "lists per wave");
-    const int lane = tid & (kWave - 1), wv = tid / kWave;
+    const int lane = tid & (kWave - 1), wv = (int)uniform_u32(tid / kWave);
     const uint2 *lbase[kListsPerWave];
-    uint32_t llen[kListsPerWave], lmax = 0;
+    uint32_t llen[kListsPerWave], lthr[kListsPerWave], lspan[kListsPerWave], lslot[kListsPerWave];
+    uint32_t lmax = 0;
 #pragma unroll
     for (int i = 0; i < kListsPerWave; i++) {
-      const int L = wv + i * kBatchWaves;
-      // the slice base is the same in every lane: move it to scalar registers, one 32-bit
-      // half at a time (the builtin takes and returns 32-bit values)
-      const uint64_t a = (uint64_t)lds.lent[L];
-      const uint32_t a_lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)a);
-      const uint32_t a_hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(a >> 32));
-      lbase[i] = (const uint2 *)(((uint64_t)a_hi << 32) | a_lo);
-      llen[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)lds.llen[L]);
+      // the list's tile slice: the same in every lane, kept in scalar registers
+      const BatchList LR = g->list[wv + i * kBatchWaves];
+      const uint32_t st = LR.dir[t], en = LR.dir[t + 1];
+      lbase[i] = uniform_ptr(LR.bent + st);
+      llen[i] = uniform_u32(en - st);
+      lthr[i] = uniform_u32(LR.thr);
+      lspan[i] = uniform_u32(LR.span);
+      lslot[i] = uniform_u32(LR.slot);
       lmax = llen[i] > lmax ? llen[i] : lmax;
     }
     for (uint32_t off = 0, first = 1; off < lmax || first; off += kWave * kListUnroll, first = 0) {
@@ -316,30 +324,39 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int s0 = 0; s0 < kSlots; s0 += kFvBatch) {
           Words<kPer> h[kFvBatch], l[kFvBatch];
+          bool use[kFvBatch];
 #pragma unroll
           for (int u = 0; u < kFvBatch; u++) {
             const BatchSlot &S = g->slot[s0 + u];
-            if (s0 + u < nslots && !S.col && w < words) {
-              h[u] = base_words<kPer>(S.fvhi, S.hibase, w, words);
-              l[u] = base_words<kPer>(S.fvlo, S.lobase, w, words);
+            use[u] = s0 + u < nslots && !uniform_ptr(S.col) && w < words;
+            if (use[u]) {
+              h[u] = base_words<kPer>(uniform_ptr(S.fvhi), uniform_u32(S.hibase), w, words);
+              l[u] = base_words<kPer>(uniform_ptr(S.fvlo), uniform_u32(S.lobase), w, words);
             }
           }
 #pragma unroll
           for (int u = 0; u < kFvBatch; u++) {
-            const BatchSlot &S = g->slot[s0 + u];
-            if (s0 + u < nslots && !S.col && w < words) {
+            if (use[u]) {
+              if constexpr (TL::kUnit == 2) {
 #pragma unroll
-              for (int k = 0; k < kPer; k++)
-                atomicXor(&lds.tog[s0 + u][k * kBatchThreads + tid], h[u].v[k] & ~l[u].v[k]);
+                for (int k = 0; k < TL::kUnits; k++) {
+                  const unsigned long long v =
+                      (unsigned long long)(h[u].v[2 * k] & ~l[u].v[2 * k]) |
+                      ((unsigned long long)(h[u].v[2 * k + 1] & ~l[u].v[2 * k + 1]) << 32);
+                  atomicXor(reinterpret_cast<unsigned long long *>(&lds.tog[s0 + u][0]) +
+                                k * kBatchThreads + tid, v);
+                }
+              } else {
+                atomicXor(&lds.tog[s0 + u][tid], h[u].v[0] & ~l[u].v[0]);
+              }
             }
           }
         }
       }
 #pragma unroll
       for (int i = 0; i < kListsPerWave; i++) {
-        const int L = wv + i * kBatchWaves;
-        const uint32_t thr = lds.lthr[L], span = lds.lspan[L];
-        uint32_t *tg = lds.tog[lds.lslot[L]];
+        const uint32_t thr = lthr[i], span = lspan[i];
+        uint32_t *tg = lds.tog[lslot[i]];
 #pragma unroll
         for (int u = 0; u < kListUnroll; u++) {
           const uint32_t e = off + u * kWave + lane;
@@ -350,8 +367,18 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
     }
     for (int s = 0; s < nslots; s++)
-      if (lds.slot[s].col) batch_direct<TW, NT>(lds.slot[s], t, n, lds.tog[s]);
+      if (g->slot[s].col) batch_direct<TW, NT>(g->slot[s], t, n, lds.tog[s]);
+    if (qload) {
+      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)Q.id * out_stride;
+      lds.qempty[qi] = Q.empty;
+      lds.qid[qi] = (uint32_t)Q.id;
+    }
+    if (tid <= nq) lds.qslot[tid] = qs;
+    // the next item's index, here rather than after the AND phase: waiting for the atomic
+    // there would also wait for all of this item's bitmap stores (one counter covers both)
+    if (tid == 0) lds.next = nblk_x + nxt;
     __syncthreads();
+    const int64_t it_next = (int64_t)lds.next;  // rewritten only after the next item's second barrier
 
     // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
     //    its own words from here on.
@@ -361,11 +388,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
       for (int k = 0; k < kPer; k++) acc[k] = fill;
       for (uint32_t s = lds.qslot[qi]; s < lds.qslot[qi + 1]; s++) {
+        if constexpr (TL::kUnit == 2) {
 #pragma unroll
-        for (int k = 0; k < kPer; k++) acc[k] &= lds.tog[s][k * kBatchThreads + tid];
+          for (int k = 0; k < TL::kUnits; k++) {
+            const uint2 v = reinterpret_cast<const uint2 *>(&lds.tog[s][0])[k * kBatchThreads + tid];
+            acc[2 * k] &= v.x;
+            acc[2 * k + 1] &= v.y;
+          }
+        } else {
+          acc[0] &= lds.tog[s][tid];
+        }
       }
       uint32_t pc = 0;
-      if (w < words) {
+      if (interior) {  // every word of the tile is a full word of the bitmap
+#pragma unroll
+        for (int k = 0; k < kPer; k++) pc += __popc(acc[k]);
+        store_words<kPer>(lds.qout[qi] + w, acc);
+      } else if (w < words) {
         BITS *o = lds.qout[qi] + w;
 #pragma unroll
         for (int k = 0; k < kPer; k++) {
@@ -384,19 +423,24 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       pc = wave_sum(pc);
       if ((tid & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
     }
-    for (int s = 0; s < nslots; s++)
+    for (int s = 0; s < nslots; s++) {
+      if constexpr (TL::kUnit == 2) {
 #pragma unroll
-      for (int k = 0; k < kPer; k++) lds.tog[s][k * kBatchThreads + tid] = 0;
+        for (int k = 0; k < TL::kUnits; k++)
+          reinterpret_cast<uint2 *>(&lds.tog[s][0])[k * kBatchThreads + tid] = make_uint2(0, 0);
+      } else {
+        lds.tog[s][tid] = 0;
+      }
+    }
     // the next item's descriptors overwrite lds.qid as soon as its loads return: read it
     // before the barrier
     const uint32_t qid = tid < nq ? lds.qid[tid] : 0;
-    if (tid == 0) lds.next = nblk_x + nxt;
     __syncthreads();
     if (tid < nq) {  // read and cleared by the same thread: the next item starts at 0
       partials[(int64_t)qid * ntiles + t] = lds.qcount[tid];
       lds.qcount[tid] = 0;
     }
-    it = (int64_t)lds.next;  // rewritten only after the next item's first barrier
+    it = it_next;
   }
   // the last workgroup to finish resets the counters for the next launch on the stream:
   // every other workgroup has taken its last item by then
