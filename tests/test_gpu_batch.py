@@ -138,3 +138,32 @@ def test_batch_autotune_and_cache(eng, tmp_path):
     assert again.astuple() == chosen.astuple()  # served from the cache
     plan.launch(out, stride, counts)
     check_results(eng, cols, queries, out, counts, stride)
+
+
+@pytest.mark.parametrize("n", [1, 31, 5000, 70001])
+def test_batch_small_tables_and_back_to_back_launches(eng, n):
+    """Tables of a few tiles leave most XCDs (and most workgroups) without work items; four
+    launches in a row, alternating launch shapes, each start from the work counters the
+    previous launch left (reset by its last workgroup) and must give the same results."""
+    if n >= 128:
+        cols, dcols, idx, sc = make_table(eng, n, seed=n % 97)
+    else:  # a BinDex needs at least 128 rows: direct evaluation only
+        cols = [eng.generate_column(k, 3, c, n, p0, p1) for c, (k, p0, p1) in enumerate(KINDS)]
+        dcols = [eng.DeviceBuffer.from_numpy(c) for c in cols]
+        idx, sc = [None] * 4, [np.sort(c) for c in cols]
+    rng = np.random.default_rng(n)
+    stride = ((n + 31) // 32 + 3) // 4 * 4
+    for nq in (1, 70):
+        queries = random_queries(rng, sc, nq, direct_share=0.2)
+        plan = eng.BatchPlan(dcols, idx, n, nq)
+        plan.set_queries(queries)
+        out = eng.DeviceBuffer(4 * stride * nq)
+        counts = eng.DeviceBuffer(8 * nq)
+        for rep, shape in enumerate([(2048, 32, 4, 1, 256), (512, 8, 2, 0, 512),
+                                     (2048, 32, 4, 1, 256), (1024, 64, 1, 1, 256)]):
+            plan.set_config(eng.batch_config(*shape))
+            out.fill(0xA5)
+            counts.fill(0xFF)
+            plan.launch(out, stride, counts)
+            check_results(eng, cols, queries, out, counts, stride)
+        plan.free()
