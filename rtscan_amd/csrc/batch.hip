@@ -102,6 +102,22 @@ struct BatchLds {
 
 constexpr int ilog2(int v) { return v <= 1 ? 0 : 1 + ilog2(v / 2); }
 
+// Device memory reached through pointers read from the group records (columns, filter
+// vectors, list entries and directories, result bitmaps) is global memory.  Saying so turns
+// the generic (flat) accesses the compiler would emit for such pointers into global ones:
+// flat accesses also count against the LDS wait counter, so every LDS wait in the kernel
+// would wait for outstanding loads and result stores as well.
+#define RTS_GLOBAL __attribute__((address_space(1)))
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+template <class T>
+__device__ __forceinline__ const RTS_GLOBAL T *gmem(const T *p) {
+  return (const RTS_GLOBAL T *)p;
+}
+template <class T>
+__device__ __forceinline__ RTS_GLOBAL T *gmem(T *p) {
+  return (RTS_GLOBAL T *)p;
+}
+
 // Toggle layout: thread t owns tile words t * kPer .. t * kPer + kPer - 1 (kPer = TW / NT)
 // as kPer / U units of U words (U = 2 when kPer >= 2: 64-bit LDS accesses), stored
 // transposed: unit k of thread t at unit position k * NT + t.  The accesses of each
@@ -136,12 +152,15 @@ __device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int6
   const int64_t tile_row0 = t * (int64_t)TW * 32;
   const int64_t wave_row0 = tile_row0 + (int64_t)wv * kWordsPerWave * 32;
   if (wave_row0 + (int64_t)kWordsPerWave * 32 <= n) {
-    const uint4 *src = reinterpret_cast<const uint4 *>(c.col + wave_row0) + lane;
+    const RTS_GLOBAL u32x4 *src = reinterpret_cast<const RTS_GLOBAL u32x4 *>(gmem(c.col) + wave_row0) + lane;
 #pragma unroll 1
     for (int s0 = 0; s0 < kSteps; s0 += kU) {
       uint4 v[kU];
 #pragma unroll
-      for (int u = 0; u < kU; u++) v[u] = load_stream(src + (s0 + u) * kWave);
+      for (int u = 0; u < kU; u++) {
+        const u32x4 x = __builtin_nontemporal_load(src + (s0 + u) * kWave);
+        v[u] = make_uint4(x.x, x.y, x.z, x.w);
+      }
 #pragma unroll
       for (int u = 0; u < kU; u++) {
         const uint32_t w = or_lanes8(nibble4(v[u], c.dlo, c.dspan) << shift);
@@ -157,7 +176,7 @@ __device__ __forceinline__ void batch_direct(const BatchSlot &c, int64_t t, int6
 #pragma unroll
     for (int j = 0; j < 4; j++) {
       const int64_t r = r0 + 4 * lane + j;
-      if (r < n && (c.col[r] - c.dlo) <= c.dspan) nib |= 8u >> j;
+      if (r < n && (gmem(c.col)[r] - c.dlo) <= c.dspan) nib |= 8u >> j;
     }
     const uint32_t w = or_lanes8(nib << shift);
     if ((lane & 7) == 0) dst[tog_index<TW, NT>(wv * kWordsPerWave + s * 8 + lane / 8)] = w;
@@ -170,7 +189,8 @@ struct Words {
 };
 
 template <int KP>
-__device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, int64_t w, int64_t words) {
+__device__ __forceinline__ Words<KP> base_words(const BITS *fv_any, uint32_t base, int64_t w, int64_t words) {
+  const RTS_GLOBAL BITS *fv = gmem(fv_any);
   Words<KP> r;
   if (base != kBaseFv) {
 #pragma unroll
@@ -181,13 +201,13 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
     if constexpr (KP == 1) {
       r.v[0] = fv[w];
     } else if constexpr (KP == 2) {
-      const uint2 x = *reinterpret_cast<const uint2 *>(fv + w);
+      const u32x2 x = *reinterpret_cast<const RTS_GLOBAL u32x2 *>(fv + w);
       r.v[0] = x.x;
       r.v[1] = x.y;
     } else {
 #pragma unroll
       for (int k = 0; k < KP; k += 4) {
-        const uint4 x = *reinterpret_cast<const uint4 *>(fv + w + k);
+        const u32x4 x = *reinterpret_cast<const RTS_GLOBAL u32x4 *>(fv + w + k);
         r.v[k] = x.x;
         r.v[k + 1] = x.y;
         r.v[k + 2] = x.z;
@@ -201,7 +221,6 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv, uint32_t base, i
   return r;
 }
 
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
 // A value that is the same in every lane, moved to a scalar register (the builtin takes and
 // returns 32-bit values: a pointer goes one half at a time).
@@ -220,15 +239,15 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
 template <int KP>
 __device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
   if constexpr (KP == 1) {
-    __builtin_nontemporal_store(acc[0], o);
+    __builtin_nontemporal_store(acc[0], gmem(o));
   } else if constexpr (KP == 2) {
     u32x2 v = {acc[0], acc[1]};
-    __builtin_nontemporal_store(v, reinterpret_cast<u32x2 *>(o));
+    __builtin_nontemporal_store(v, reinterpret_cast<RTS_GLOBAL u32x2 *>(gmem(o)));
   } else {
 #pragma unroll
     for (int k = 0; k < KP; k += 4) {
       u32x4 v = {acc[k], acc[k + 1], acc[k + 2], acc[k + 3]};
-      __builtin_nontemporal_store(v, reinterpret_cast<u32x4 *>(o + k));
+      __builtin_nontemporal_store(v, reinterpret_cast<RTS_GLOBAL u32x4 *>(gmem(o) + k));
     }
   }
 }
@@ -302,7 +321,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int i = 0; i < kListsPerWave; i++) {
       // the list's tile slice: the same in every lane, kept in scalar registers
       const BatchList LR = g->list[wv + i * kBatchWaves];
-      const uint32_t st = LR.dir[t], en = LR.dir[t + 1];
+      const uint32_t st = gmem(LR.dir)[t], en = gmem(LR.dir)[t + 1];
       lbase[i] = uniform_ptr(LR.bent + st);
       llen[i] = uniform_u32(en - st);
       lthr[i] = uniform_u32(LR.thr);
@@ -317,7 +336,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int u = 0; u < kListUnroll; u++) {
           const uint32_t e = off + u * kWave + lane;
-          ent[i][u] = e < llen[i] ? lbase[i][e] : make_uint2(0, 0);
+          u32x2 x = {0u, 0u};
+          if (e < llen[i]) x = reinterpret_cast<const RTS_GLOBAL u32x2 *>(gmem(lbase[i]))[e];
+          ent[i][u] = make_uint2(x.x, x.y);
         }
       if (first) {
         constexpr int kFvBatch = kPer == 1 ? 8 : 16 / kPer;  // slots whose words are in flight together
@@ -417,7 +438,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         } else {
 #pragma unroll
           for (int k = 0; k < kPer; k++)
-            if (w + k < words) o[k] = acc[k];
+            if (w + k < words) gmem(o)[k] = acc[k];
         }
       }
       pc = wave_sum(pc);
