@@ -13,15 +13,16 @@
 // same tile or two: the tile's filter-vector and list slices are read from HBM once and,
 // as far as the 4 MB L2 holds them, re-read by the other query groups from there.  (A
 // static round-robin of the same order lets the workgroups drift over several tiles, whose
-// slices together overflow the L2; it ran 4-18% slower.)  Popcounts go to one partial per (query, tile), summed by a second
-// small kernel.
+// slices together overflow the L2; it ran 4-18% slower.)  Popcounts go to one partial per
+// (query, tile), summed by a second small kernel.
 //
-// An item is latency-bound unless many loads are in flight, so each phase issues its loads
-// together: a group is one contiguous record (slots, lists, queries) loaded in one round,
-// then the lists' directory words; refine entries are list-major (wave v owns lists v,
-// v + 8, ...; a list's slice base is wave-uniform), 16 entries per lane in flight; the
-// filter-vector words go out with the first refine round and are XORed into the LDS
-// toggles as they arrive.
+// An item is a chain of dependent memory round trips (group record, directory words, list
+// entries and filter-vector words) and two barriers, so the launch is bound by latency:
+// each wave reads its own lists' records and directory words (wave-uniform addresses, no
+// exchange through LDS); refine entries are list-major (wave v of W owns lists v, v + W,
+// ...), 16 entries per lane in flight; the filter-vector words go out with the first refine
+// round and are XORed into the LDS toggles as they arrive; the query descriptors reach LDS
+// before the one barrier that precedes the AND phase.
 #include <string.h>
 
 #include <algorithm>
@@ -307,9 +308,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time
     //    (16 at 512-word tiles), XORed into the toggles as they arrive: every toggle update
     //    is an XOR, so the order does not matter and the words need not stay in registers.
-    //    Refine entries: wave v owns lists v, v + 8, ...; a list's slice base is uniform
-    //    across the wave, each lane takes kListUnroll entries of each of its lists per
-    //    round, and the first round's loads go out together with the filter-vector loads.
+    //    Refine entries: wave v of W owns lists v, v + W, ...; a list's slice base is
+    //    uniform across the wave, each lane takes kListUnroll entries of each of its lists
+    //    per round, and the first round's loads go out together with the filter-vector
+    //    loads.
     constexpr int kListsPerWave = kLists / kBatchWaves;
     constexpr int kListUnroll = 16 / kListsPerWave;
     static_assert(kLists % kBatchWaves == 0, "lists per wave");
@@ -399,7 +401,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     // there would also wait for all of this item's bitmap stores (one counter covers both)
     if (tid == 0) lds.next = nblk_x + nxt;
     __syncthreads();
-    const int64_t it_next = (int64_t)lds.next;  // rewritten only after the next item's second barrier
+    // thread 0 writes lds.next again only in the next item, past this item's end barrier
+    const int64_t it_next = (int64_t)lds.next;
 
     // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
     //    its own words from here on.
@@ -453,8 +456,8 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         lds.tog[s][tid] = 0;
       }
     }
-    // the next item's descriptors overwrite lds.qid as soon as its loads return: read it
-    // before the barrier
+    // past the barrier, faster waves may already store the next item's query descriptors:
+    // read qid before it
     const uint32_t qid = tid < nq ? lds.qid[tid] : 0;
     __syncthreads();
     if (tid < nq) {  // read and cleared by the same thread: the next item starts at 0
