@@ -118,6 +118,14 @@ template <class T>
 __device__ __forceinline__ RTS_GLOBAL T *gmem(T *p) {
   return (RTS_GLOBAL T *)p;
 }
+// Memory that no kernel writes while a batch runs (the lists' tile directories), read
+// through the constant address space: a wave-uniform address then becomes a scalar load,
+// whose wait does not also wait for the wave's outstanding result stores.
+#define RTS_CONST __attribute__((address_space(4)))
+template <class T>
+__device__ __forceinline__ const RTS_CONST T *cmem(const T *p) {
+  return (const RTS_CONST T *)p;
+}
 
 // Toggle layout: thread t owns tile words t * kPer .. t * kPer + kPer - 1 (kPer = TW / NT)
 // as kPer / U units of U words (U = 2 when kPer >= 2: 64-bit LDS accesses), stored
@@ -189,6 +197,36 @@ struct Words {
   uint32_t v[KP];
 };
 
+// KP filter-vector words from w on, every one of them inside the bitmap: vector loads only.
+template <int KP>
+__device__ __forceinline__ Words<KP> base_words_whole(const BITS *fv_any, uint32_t base, int64_t w) {
+  const RTS_GLOBAL BITS *fv = gmem(fv_any);
+  Words<KP> r;
+  if (base != kBaseFv) {
+#pragma unroll
+    for (int k = 0; k < KP; k++) r.v[k] = base == kBaseOnes ? ~0u : 0u;
+    return r;
+  }
+  if constexpr (KP == 1) {
+    r.v[0] = fv[w];
+  } else if constexpr (KP == 2) {
+    const u32x2 x = *reinterpret_cast<const RTS_GLOBAL u32x2 *>(fv + w);
+    r.v[0] = x.x;
+    r.v[1] = x.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < KP; k += 4) {
+      const u32x4 x = *reinterpret_cast<const RTS_GLOBAL u32x4 *>(fv + w + k);
+      r.v[k] = x.x;
+      r.v[k + 1] = x.y;
+      r.v[k + 2] = x.z;
+      r.v[k + 3] = x.w;
+    }
+  }
+  return r;
+}
+
+// As base_words_whole, with the words past the bitmap's end read as 0 (the last tile).
 template <int KP>
 __device__ __forceinline__ Words<KP> base_words(const BITS *fv_any, uint32_t base, int64_t w, int64_t words) {
   const RTS_GLOBAL BITS *fv = gmem(fv_any);
@@ -253,6 +291,53 @@ __device__ __forceinline__ void store_words(BITS *o, const uint32_t *acc) {
   }
 }
 
+// Filter-vector words of the group's BinDex slots, XORed into their toggles: 32 words per
+// thread in flight at a time (16 at 512-word tiles), XORed as they arrive (every toggle
+// update is an XOR, so the order does not matter and the words need not stay in
+// registers).  kWhole: every word of the tile lies inside the bitmap.
+template <int TW, int NT, bool kWhole, class Lds>
+__device__ __forceinline__ void xor_filter_vectors(Lds &lds, const BatchGroup *g, int nslots, int64_t w,
+                                                   int64_t words, int tid) {
+  constexpr int kPer = TW / NT;
+  using TL = TogLayout<TW, NT>;
+  constexpr int kSlots = Lds::kSlots;
+  constexpr int kFvBatch = kPer == 1 ? 8 : 16 / kPer;  // slots whose words are in flight together
+#pragma unroll
+  for (int s0 = 0; s0 < kSlots; s0 += kFvBatch) {
+    Words<kPer> h[kFvBatch], l[kFvBatch];
+    bool use[kFvBatch];
+#pragma unroll
+    for (int u = 0; u < kFvBatch; u++) {
+      const BatchSlot S = g->slot[s0 + u];
+      use[u] = s0 + u < nslots && !uniform_ptr(S.col) && (kWhole || w < words);
+      if (use[u]) {
+        if constexpr (kWhole) {
+          h[u] = base_words_whole<kPer>(uniform_ptr(S.fvhi), uniform_u32(S.hibase), w);
+          l[u] = base_words_whole<kPer>(uniform_ptr(S.fvlo), uniform_u32(S.lobase), w);
+        } else {
+          h[u] = base_words<kPer>(uniform_ptr(S.fvhi), uniform_u32(S.hibase), w, words);
+          l[u] = base_words<kPer>(uniform_ptr(S.fvlo), uniform_u32(S.lobase), w, words);
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kFvBatch; u++) {
+      if (use[u]) {
+        if constexpr (TL::kUnit == 2) {
+#pragma unroll
+          for (int k = 0; k < TL::kUnits; k++) {
+            const unsigned long long v = (unsigned long long)(h[u].v[2 * k] & ~l[u].v[2 * k]) |
+                                         ((unsigned long long)(h[u].v[2 * k + 1] & ~l[u].v[2 * k + 1]) << 32);
+            atomicXor(reinterpret_cast<unsigned long long *>(&lds.tog[s0 + u][0]) + k * NT + tid, v);
+          }
+        } else {
+          atomicXor(&lds.tog[s0 + u][tid], h[u].v[0] & ~l[u].v[0]);
+        }
+      }
+    }
+  }
+}
+
 template <int TW, int NT>
 __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_batch(
     const BatchGroup *__restrict__ groups, const int *__restrict__ ngroups_ptr, int64_t n,
@@ -305,13 +390,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 
     const int64_t w = t * TW + lw;
     const bool interior = (t + 1) * TW < words;  // the last bitmap word lies in a later tile
-    // 2. Filter-vector words of the BinDex slots, 32 words per thread in flight at a time
-    //    (16 at 512-word tiles), XORed into the toggles as they arrive: every toggle update
-    //    is an XOR, so the order does not matter and the words need not stay in registers.
-    //    Refine entries: wave v of W owns lists v, v + W, ...; a list's slice base is
-    //    uniform across the wave, each lane takes kListUnroll entries of each of its lists
-    //    per round, and the first round's loads go out together with the filter-vector
-    //    loads.
+    // 2. Filter-vector words of the BinDex slots (xor_filter_vectors) and refine entries:
+    //    wave v of W owns lists v, v + W, ...; a list's slice base is uniform across the
+    //    wave, each lane takes kListUnroll entries of each of its lists per round, and the
+    //    first round's loads go out together with the filter-vector loads.
     constexpr int kListsPerWave = kLists / kBatchWaves;
     constexpr int kListUnroll = 16 / kListsPerWave;
     static_assert(kLists % kBatchWaves == 0, "lists per wave");
@@ -323,7 +405,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int i = 0; i < kListsPerWave; i++) {
       // the list's tile slice: the same in every lane, kept in scalar registers
       const BatchList LR = g->list[wv + i * kBatchWaves];
-      const uint32_t st = gmem(LR.dir)[t], en = gmem(LR.dir)[t + 1];
+      const uint32_t st = cmem(LR.dir)[t], en = cmem(LR.dir)[t + 1];
       lbase[i] = uniform_ptr(LR.bent + st);
       llen[i] = uniform_u32(en - st);
       lthr[i] = uniform_u32(LR.thr);
@@ -343,38 +425,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
           ent[i][u] = make_uint2(x.x, x.y);
         }
       if (first) {
-        constexpr int kFvBatch = kPer == 1 ? 8 : 16 / kPer;  // slots whose words are in flight together
-#pragma unroll
-        for (int s0 = 0; s0 < kSlots; s0 += kFvBatch) {
-          Words<kPer> h[kFvBatch], l[kFvBatch];
-          bool use[kFvBatch];
-#pragma unroll
-          for (int u = 0; u < kFvBatch; u++) {
-            const BatchSlot &S = g->slot[s0 + u];
-            use[u] = s0 + u < nslots && !uniform_ptr(S.col) && w < words;
-            if (use[u]) {
-              h[u] = base_words<kPer>(uniform_ptr(S.fvhi), uniform_u32(S.hibase), w, words);
-              l[u] = base_words<kPer>(uniform_ptr(S.fvlo), uniform_u32(S.lobase), w, words);
-            }
-          }
-#pragma unroll
-          for (int u = 0; u < kFvBatch; u++) {
-            if (use[u]) {
-              if constexpr (TL::kUnit == 2) {
-#pragma unroll
-                for (int k = 0; k < TL::kUnits; k++) {
-                  const unsigned long long v =
-                      (unsigned long long)(h[u].v[2 * k] & ~l[u].v[2 * k]) |
-                      ((unsigned long long)(h[u].v[2 * k + 1] & ~l[u].v[2 * k + 1]) << 32);
-                  atomicXor(reinterpret_cast<unsigned long long *>(&lds.tog[s0 + u][0]) +
-                                k * kBatchThreads + tid, v);
-                }
-              } else {
-                atomicXor(&lds.tog[s0 + u][tid], h[u].v[0] & ~l[u].v[0]);
-              }
-            }
-          }
-        }
+        // a wave-uniform choice: interior tiles load whole vectors with no per-lane bounds
+        if (interior)
+          xor_filter_vectors<TW, NT, true>(lds, g, nslots, w, words, tid);
+        else
+          xor_filter_vectors<TW, NT, false>(lds, g, nslots, w, words, tid);
       }
 #pragma unroll
       for (int i = 0; i < kListsPerWave; i++) {
