@@ -49,11 +49,11 @@ struct BatchShape {
 // the slot's refine lists; bases kBaseZero / kBaseOnes need no filter vector.  Direct form
 // (col non-null): the words of (x - dlo) <= dspan.
 struct BatchSlot {
-  const BITS *fvhi;
+  const BITS *fvhi;  // the filter-vector fields first: one 32-byte scalar load
   const BITS *fvlo;
   const CODE *col;
-  CODE dlo, dspan;
   uint32_t hibase, lobase;
+  CODE dlo, dspan;
 };
 
 // One refine list of a group: the rows of a bin's B list (in row order, tile directory
