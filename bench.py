@@ -12,15 +12,17 @@ fraction (algorithmic bytes per launch / measured time per launch), and the CPU 
 (the oracle's multithreaded port of the reference raw_scan, timed on this host on a
 bounded sample of the same predicates).  `extra` carries the other configurations: the
 BinDex query of the same predicates, conjunctions over the 1e8 x 4 table, 4096 batched
-queries, and the request loop over resident 2^26 x 2 and 1e9 x 4 tables (p50 / p99).
+queries, the paths either side of the scan (remap encoders, result checker and digest,
+pinned result egress, index snapshots), and the request loop over resident 2^26 x 2 and
+1e9 x 4 tables (p50 / p99).
 """
 import argparse
 import gc
 import json
 import os
 import sys
+import tempfile
 import time
-import types
 
 import numpy as np
 
@@ -275,7 +277,7 @@ def bench_conj(R, table, nq=30, seed=33):
     out = R.DeviceBuffer(4 * W)
     tmp = [R.DeviceBuffer(4 * W) for _ in range(4)]
     cnts = R.DeviceBuffer(8 * nq)  # one count slot per query
-    slot = [types.SimpleNamespace(ptr=cnts.ptr + 8 * i) for i in range(nq)]
+    slot = [_Slot(cnts, 8 * i) for i in range(nq)]
     res = {}
     for k in (2, 3, 4):
         qs = []
@@ -346,6 +348,89 @@ def bench_conj(R, table, nq=30, seed=33):
         truth = oracle.conjunction([table.cols[c] for c in cs], preds, threads=16)
         ok = ok and bool(np.array_equal(out.to_numpy(np.uint32, W), truth))
     res["oracle_bitmaps_match"] = ok
+    return res
+
+
+def bench_aux(R, n=N_ROWS, snap_rows=1 << 24, reps=3):
+    """The paths either side of the scan on the 1e8-row uniform column: the remap encoders
+    (rank and dispersed forms of normalEncode, remap.cpp:110-241; wall clock of the C entry
+    point, which returns after the map size is known), the device result checker and digest
+    over a 1e8-row bitmap (HIP events), result egress into pinned host memory (HIP events),
+    and BinDex snapshot save / load for a 2^24-row column (wall clock, a temporary file)."""
+    import ctypes
+
+    L = R.lib()
+    x = R.generate_column(R.DIST_UNIFORM, DATA_SEED, 0, n)
+    dx = R.DeviceBuffer.from_numpy(x)
+    del x
+    stream = R.Stream()
+    res = {}
+    codes, mv, ms = (R.DeviceBuffer(4 * n) for _ in range(3))
+    m = ctypes.c_int64()
+
+    def wall(fn):
+        fn()  # warm-up
+        t = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            t.append(time.perf_counter() - t0)
+        return float(np.median(t)) * 1e3
+
+    enc_ms = wall(lambda: R.check(L.rts_encode_column(dx.ptr, n, 0, 0xFFFFFFFF, codes.ptr, None, None,
+                                                      mv.ptr, ms.ptr, ctypes.byref(m), stream.handle),
+                                  "rts_encode_column"))
+    res["encode_rank"] = {"ms": enc_ms, "rows_per_s": n / enc_ms * 1e3, "distinct": m.value}
+    disp_ms = wall(lambda: R.check(L.rts_encode_column_dispersed(dx.ptr, n, 0, 0xFFFFFFFF, 7, codes.ptr,
+                                                                 mv.ptr, ms.ptr, ctypes.byref(m),
+                                                                 stream.handle),
+                                   "rts_encode_column_dispersed"))
+    res["encode_dispersed"] = {"ms": disp_ms, "rows_per_s": n / disp_ms * 1e3}
+    for b in (codes, mv, ms):
+        b.free()
+
+    W = (n + 31) // 32
+    a, b = R.DeviceBuffer(4 * W), R.DeviceBuffer(4 * W)
+    R.scan_range(dx, n, 1 << 30, 3 << 30, a, None, stream)
+    R.scan_range(dx, n, 1 << 30, 3 << 30, b, None, stream)
+    stream.synchronize()
+    steps = 20
+    dg = ctypes.c_ulonglong()
+    cr = R._CompareResult()
+    dig = timed_loop(R, stream, steps, 2,
+                     lambda i: R.check(L.rts_bitmap_digest(a.ptr, n, ctypes.byref(dg), stream.handle),
+                                       "rts_bitmap_digest")) / steps
+    cmp_ = timed_loop(R, stream, steps, 2,
+                      lambda i: R.check(L.rts_bitmap_compare(a.ptr, b.ptr, n, ctypes.byref(cr),
+                                                             stream.handle), "rts_bitmap_compare")) / steps
+    res["digest"] = {"us": dig * 1e3, "GBps": 4 * W / (dig * 1e-3) / 1e9}
+    res["compare"] = {"us": cmp_ * 1e3, "GBps": 8 * W / (cmp_ * 1e-3) / 1e9, "equal": cr.first_mismatch == -1}
+    host = ctypes.c_void_p()
+    R.check(L.rts_host_alloc(ctypes.byref(host), 4 * W), "rts_host_alloc")
+    egress = timed_loop(R, stream, 10, 2,
+                        lambda i: R.check(L.rts_memcpy_dtoh_async(host.value, a.ptr, 4 * W, stream.handle),
+                                          "rts_memcpy_dtoh_async")) / 10
+    L.rts_host_free(host.value)
+    res["egress_pinned"] = {"ms": egress, "GBps": 4 * W / (egress * 1e-3) / 1e9, "bytes": 4 * W}
+    a.free()
+    b.free()
+
+    ix = R.BinDexIndex(_Slot(dx, 0), snap_rows)
+    nbytes = ix.device_bytes()
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "index.rtsbdx")
+        t0 = time.perf_counter()
+        ix.save(path)
+        save_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ix2 = R.BinDexIndex(_Slot(dx, 0), snap_rows, snapshot=path)
+        load_s = time.perf_counter() - t0
+        fsize = os.path.getsize(path)
+        ix2.free()
+    ix.free()
+    res["snapshot"] = {"rows": snap_rows, "file_bytes": fsize, "device_bytes": nbytes,
+                       "save_GBps": fsize / save_s / 1e9, "load_GBps": fsize / load_s / 1e9}
+    dx.free()
     return res
 
 
@@ -554,7 +639,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="scan",
-                    choices=["scan", "copycal", "conj", "batch", "serve"])
+                    choices=["scan", "copycal", "conj", "batch", "serve", "aux"])
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
@@ -575,6 +660,8 @@ def main():
         sys.exit(1)
     if args.workload == "copycal":
         res = bench_copycal(args, R)
+    elif args.workload == "aux":
+        res = {"aux": bench_aux(R)}
     elif args.workload == "serve":
         cfg = tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config else None
         res = {"serve": bench_serve(R, n2=args.serve_rows, batch_config=cfg)}
@@ -599,6 +686,11 @@ def main():
             except Exception as e:  # noqa: BLE001
                 res["extra"]["error"] = "%s: %s" % (type(e).__name__, e)
             gc.collect()  # the 1e8 x 4 table and the batch outputs leave HBM before serving
+            try:
+                res["extra"]["aux"] = bench_aux(R)
+            except Exception as e:  # noqa: BLE001
+                res["extra"]["aux_error"] = "%s: %s" % (type(e).__name__, e)
+            gc.collect()
             try:
                 res["extra"]["serve"] = bench_serve(R, n2=args.serve_rows)
             except Exception as e:  # noqa: BLE001

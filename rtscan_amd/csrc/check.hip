@@ -9,6 +9,10 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <tuple>
 #include <vector>
 
 #include "common.h"
@@ -117,10 +121,39 @@ void grid_for(uint64_t words, unsigned *blocks, uint64_t *chunk) {
   *blocks = (unsigned)((words + *chunk - 1) / *chunk);
 }
 
-template <class T>
-hipError_t run_parts(unsigned blocks, std::vector<T> &host, T **dev) {
-  host.resize(blocks);
-  return hipMalloc((void **)dev, blocks * sizeof(T));
+// Per-block partials of one check, on the device and in pinned host memory, kept per
+// (device, stream, host thread) so that a check allocates nothing after its first call
+// (hipFree would also synchronise the device) and two threads never share them.
+struct CheckScratch {
+  void *dev = nullptr;
+  void *host = nullptr;
+};
+
+constexpr size_t kCheckScratchBytes = kCheckMaxBlocks * sizeof(ComparePart);
+
+hipError_t check_scratch(hipStream_t stream, CheckScratch *out) {
+  static std::mutex mu;
+  static std::map<std::tuple<int, uintptr_t, std::thread::id>, CheckScratch> table;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  const auto key = std::make_tuple(dev, (uintptr_t)stream, std::this_thread::get_id());
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = table.find(key);
+  if (it != table.end()) {
+    *out = it->second;
+    return hipSuccess;
+  }
+  CheckScratch c;
+  e = hipMalloc(&c.dev, kCheckScratchBytes);
+  if (e == hipSuccess) e = hipHostMalloc(&c.host, kCheckScratchBytes, 0);
+  if (e != hipSuccess) {
+    if (c.dev) (void)hipFree(c.dev);
+    return e;
+  }
+  table[key] = c;
+  *out = c;
+  return hipSuccess;
 }
 
 }  // namespace
@@ -131,18 +164,19 @@ hipError_t bitmap_compare(const BITS *truth, const BITS *got, int64_t n, rts_com
   unsigned blocks;
   uint64_t chunk;
   grid_for(words, &blocks, &chunk);
-  std::vector<ComparePart> parts;
-  ComparePart *d_parts = nullptr;
-  hipError_t e = run_parts(blocks, parts, &d_parts);
+  CheckScratch sc;
+  hipError_t e = check_scratch(s, &sc);
   if (e != hipSuccess) return e;
+  ComparePart *d_parts = static_cast<ComparePart *>(sc.dev);
   hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(kCheckBlock), 0, s, truth, got, words, tail_mask((uint64_t)n),
                      chunk, d_parts);
   e = hipGetLastError();
   if (e == hipSuccess)
-    e = hipMemcpyAsync(parts.data(), d_parts, blocks * sizeof(ComparePart), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(sc.host, d_parts, blocks * sizeof(ComparePart), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(d_parts);
   if (e != hipSuccess) return e;
+  const std::vector<ComparePart> parts(static_cast<const ComparePart *>(sc.host),
+                                       static_cast<const ComparePart *>(sc.host) + blocks);
   rts_compare_result r{0, 0, -1, 0, 0};
   bool stopped = false;
   for (const ComparePart &p : parts) {
@@ -184,18 +218,19 @@ hipError_t bitmap_digest(const BITS *bm, int64_t n, unsigned long long *digest, 
   unsigned blocks;
   uint64_t chunk;
   grid_for(words, &blocks, &chunk);
-  std::vector<unsigned long long> parts;
-  unsigned long long *d_parts = nullptr;
-  hipError_t e = run_parts(blocks, parts, &d_parts);
+  CheckScratch sc;
+  hipError_t e = check_scratch(s, &sc);
   if (e != hipSuccess) return e;
+  unsigned long long *d_parts = static_cast<unsigned long long *>(sc.dev);
   hipLaunchKernelGGL(k_digest, dim3(blocks), dim3(kCheckBlock), 0, s, bm, words, tail_mask((uint64_t)n), chunk,
                      d_parts);
   e = hipGetLastError();
   if (e == hipSuccess)
-    e = hipMemcpyAsync(parts.data(), d_parts, blocks * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(sc.host, d_parts, blocks * sizeof(unsigned long long), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(d_parts);
   if (e != hipSuccess) return e;
+  const std::vector<unsigned long long> parts(static_cast<const unsigned long long *>(sc.host),
+                                              static_cast<const unsigned long long *>(sc.host) + blocks);
   unsigned long long d = 0;
   for (unsigned long long p : parts) d += p;
   *digest = d;
