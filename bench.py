@@ -96,7 +96,33 @@ def cpu_baseline(x, preds, threads, budget_s=1.5, max_preds=24):
         if t_total >= budget_s:
             break
     return {"rows_per_s": done * n / t_total, "preds": done, "wall_s": t_total,
-            "counts": counts}
+            "counts": counts, "reference": reference_raw_scan(x, preds, counts)}
+
+
+def reference_raw_scan(x, preds, port_counts, budget_s=1.0, max_preds=4):
+    """The reference's own raw_scan (bindex.cpp:1851-1892, compiled from its sources into
+    oracle/_ref; one host thread, as the reference runs it) on the first predicates over
+    the full column, until ~budget_s of wall time; None when that library is not built."""
+    import oracle
+
+    try:
+        ref = oracle.RefBindexLib(524288)  # raw_scan reads the row count at run time
+    except (FileNotFoundError, OSError):
+        return None
+    n = x.size
+    done, t_total, same = 0, 0.0, True
+    for i, (lo, hi) in enumerate(preds[:max_preds]):
+        op, t1, t2 = reference_predicate(lo, hi)
+        t0 = time.perf_counter()
+        bm = ref.raw_scan(x, op, t1, t2)
+        t_total += time.perf_counter() - t0
+        if i < len(port_counts):
+            same = same and oracle.popcount_rows(bm, n) == port_counts[i]
+        done += 1
+        if t_total >= budget_s:
+            break
+    return {"rows_per_s": done * n / t_total, "preds": done, "wall_s": t_total, "threads": 1,
+            "counts_match_port": bool(same)}
 
 
 class _Slot:
@@ -222,6 +248,8 @@ def bench_scan(args, R):
             "bindex_query_rows_per_s": n * steps / (ms_b / 1e3),
             "bindex_query_alg_GBps": b_alg / (ms_b / 1e3 / steps) / 1e9,
             "bindex_device_bytes": ix.device_bytes(),
+            # the reference's compiled raw_scan itself on this host, beside the port above
+            "cpu_reference_raw_scan": cpu["reference"],
         },
         "parity": {"checked_predicates": len(cpu["counts"]), "popcounts_match_oracle": bool(ok),
                    "bindex_counts_match_scan": bq_ok},
