@@ -41,14 +41,14 @@ struct QueryArgs {
 };
 
 // Rows of bound b's refine list in tile t with value in [thr, thr + span] (a bound: value
-// >= thr; a narrow column: its whole range): marked in the tile's LDS toggles.  The list's
+// >= thr; a narrow column: its whole range): toggled in the tile's LDS toggles.  The list's
 // tile slice [e0, e1) was read from the directory beforehand.
 __device__ __forceinline__ void mark_removed(const BoundDev &b, uint32_t e0, uint32_t e1, int64_t t,
                                              uint32_t *tog) {
   const uint32_t row0 = (uint32_t)(t * kTileRows);
   for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
     const uint2 pv = b.bent[e];
-    if (pv.y - b.thr <= b.span) atomicOr(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
+    if (pv.y - b.thr <= b.span) atomicXor(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
   }
 }
 
@@ -67,7 +67,11 @@ template <int C, int MODE, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t n, int64_t words,
                                                          int64_t ntiles, BITS *out, bool out_vec,
                                                          CountSink sink) {
-  __shared__ uint32_t tog[C][2][kTileWords];
+  // One toggle array per column for both bounds: with L(v) = fv ^ t_v the rows below v,
+  // L(lo) is inside L(hi + 1), so L(hi + 1) & ~L(lo) = (fv_hi ^ fv_lo) ^ (t_lo ^ t_hi); the two
+  // bounds' lists cover different bins (or, both bounds in one bin, a single list) and
+  // toggle disjoint rows of the same array.
+  __shared__ alignas(16) uint32_t tog[C][kTileWords];
   constexpr int kPer = kTileWords / kBlock;  // 4 words per thread
   static_assert(kPer == 4, "one uint4 of output words per thread");
   const uint32_t last_mask = tail_mask((uint64_t)n);
@@ -96,12 +100,12 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
         hi_e[c][1] = q.hi[c].dir[t + 1];
       }
     }
-    for (int i = threadIdx.x; i < C * 2 * kTileWords; i += kBlock) (&tog[0][0][0])[i] = 0;
+    for (int i = threadIdx.x; i < C * kTileWords; i += kBlock) (&tog[0][0])[i] = 0;
     __syncthreads();
 #pragma unroll
     for (int c = 0; c < C; c++) {
-      mark_removed(q.lo[c], lo_e[c][0], lo_e[c][1], t, tog[c][0]);
-      mark_removed(q.hi[c], hi_e[c][0], hi_e[c][1], t, tog[c][1]);
+      mark_removed(q.lo[c], lo_e[c][0], lo_e[c][1], t, tog[c]);
+      mark_removed(q.hi[c], hi_e[c][0], hi_e[c][1], t, tog[c]);
     }
     __syncthreads();
     if (w < words) {
@@ -109,11 +113,11 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
 #pragma unroll
       for (int c = 0; c < C; c++) {
         const uint4 lo = lo_w[c], hi = hi_w[c];
-        const uint32_t *tl = &tog[c][0][lw], *th = &tog[c][1][lw];
-        acc.x &= (hi.x ^ th[0]) & ~(lo.x ^ tl[0]);
-        acc.y &= (hi.y ^ th[1]) & ~(lo.y ^ tl[1]);
-        acc.z &= (hi.z ^ th[2]) & ~(lo.z ^ tl[2]);
-        acc.w &= (hi.w ^ th[3]) & ~(lo.w ^ tl[3]);
+        const uint4 tg = *reinterpret_cast<const uint4 *>(&tog[c][lw]);
+        acc.x &= hi.x ^ lo.x ^ tg.x;
+        acc.y &= hi.y ^ lo.y ^ tg.y;
+        acc.z &= hi.z ^ lo.z ^ tg.z;
+        acc.w &= hi.w ^ lo.w ^ tg.w;
       }
       uint32_t v[4] = {acc.x, acc.y, acc.z, acc.w};
 #pragma unroll
@@ -190,8 +194,8 @@ static hipError_t query_group(const BinDex *const *bxs, const CODE *lo, const CO
     q.lo[c] = make_bound(d, sl);
     q.hi[c] = make_bound(d, sh);
     if (sl.list >= 0 && sl.list == sh.list) {
-      // both bounds in one bin: (1 ^ 0) & ~(1 ^ t) = t with t the bin's rows in [lo, hi],
-      // one pass over the list and no filter vector
+      // both bounds in one bin: (1 ^ 1) ^ t = t with t the bin's rows in [lo, hi], one pass
+      // over the list and no filter vector
       q.lo[c].base = q.hi[c].base = kBaseOnes;
       q.lo[c].span = hi[c] - lo[c];
       q.hi[c].dir = nullptr;
