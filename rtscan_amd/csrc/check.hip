@@ -22,7 +22,7 @@ namespace rts {
 namespace {
 
 constexpr int kCheckBlock = 256;
-constexpr unsigned kCheckMaxBlocks = 1024;
+constexpr unsigned kCheckMaxBlocks = 4096;
 constexpr uint64_t kNone = ~0ull;
 
 struct ComparePart {
@@ -67,6 +67,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_compare(const BITS *truth, cons
   const uint64_t w0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t w1 = w0 + chunk < words ? w0 + chunk : words;
   unsigned long long pt = 0, pb = 0, pg = 0, fm = kNone;
+#pragma unroll 4
   for (uint64_t w = w0 + threadIdx.x; w < w1; w += kCheckBlock) {
     const uint32_t a = word_at(truth, w, words, last), b = word_at(got, w, words, last);
     pt += __popc(a);
@@ -95,6 +96,70 @@ __global__ __launch_bounds__(kCheckBlock) void k_compare(const BITS *truth, cons
   if (threadIdx.x == 0) parts[blockIdx.x] = ComparePart{t, bo, g, tu, bu, first};
 }
 
+// The per-block parts of k_compare folded into compare_bitmap's result on the device, so
+// that only the result crosses to the host: counts over all blocks; hits up to the first
+// block with a mismatch (that block up to its first differing row) or over all blocks; the
+// first differing row from its word pair.  One workgroup of kFinalThreads.
+constexpr int kFinalThreads = 1024;
+
+__device__ unsigned long long final_sum(unsigned long long v, unsigned long long *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  unsigned long long t = 0;
+  for (int i = 0; i < kFinalThreads / kWave; i++) t += sh[i];
+  return t;
+}
+
+__device__ unsigned long long final_min(unsigned long long v, unsigned long long *sh) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long u = __shfl_xor(v, o, kWave);
+    v = u < v ? u : v;
+  }
+  __syncthreads();
+  if ((threadIdx.x & (kWave - 1)) == 0) sh[threadIdx.x / kWave] = v;
+  __syncthreads();
+  unsigned long long t = kNone;
+  for (int i = 0; i < kFinalThreads / kWave; i++) t = sh[i] < t ? sh[i] : t;
+  return t;
+}
+
+__global__ __launch_bounds__(kFinalThreads) void k_compare_finalize(const ComparePart *parts, unsigned blocks,
+                                                                    const BITS *truth, const BITS *got,
+                                                                    uint64_t words, uint32_t last,
+                                                                    rts_compare_result *out) {
+  __shared__ unsigned long long sh[kFinalThreads / kWave];
+  unsigned long long t = 0, g = 0, fb = kNone;
+  for (unsigned i = threadIdx.x; i < blocks; i += kFinalThreads) {
+    t += parts[i].truth;
+    g += parts[i].got;
+    if (parts[i].first_word != kNone && i < fb) fb = i;
+  }
+  const unsigned long long truth_count = final_sum(t, sh), got_count = final_sum(g, sh);
+  const unsigned long long first_block = final_min(fb, sh);
+  const unsigned long long upto = first_block == kNone ? blocks : first_block;  // whole blocks
+  unsigned long long ht = 0, hb = 0;
+  for (unsigned i = threadIdx.x; i < upto; i += kFinalThreads) {
+    ht += parts[i].truth;
+    hb += parts[i].both;
+  }
+  const unsigned long long total_hit = final_sum(ht, sh), true_hit = final_sum(hb, sh);
+  if (threadIdx.x == 0) {
+    rts_compare_result r{total_hit, true_hit, -1, truth_count, got_count};
+    if (first_block != kNone) {
+      const ComparePart &p = parts[first_block];
+      r.total_hit += p.truth_upto;
+      r.true_hit += p.both_upto;
+      const uint32_t a = word_at(truth, p.first_word, words, last), b = word_at(got, p.first_word, words, last);
+      r.first_mismatch = (long long)(p.first_word * 32 + (uint64_t)__clz(a ^ b));
+    }
+    *out = r;
+  }
+}
+
 __device__ __forceinline__ unsigned long long mix64(unsigned long long x) {
   x ^= 0x9E3779B97F4A7C15ull;
   x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -108,6 +173,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_digest(const BITS *bm, uint64_t
   const uint64_t w0 = (uint64_t)blockIdx.x * chunk;
   const uint64_t w1 = w0 + chunk < words ? w0 + chunk : words;
   unsigned long long d = 0;
+#pragma unroll 4
   for (uint64_t w = w0 + threadIdx.x; w < w1; w += kCheckBlock)
     d += mix64((w << 32) | word_at(bm, w, words, last));
   d = block_sum(d, sh);
@@ -115,7 +181,7 @@ __global__ __launch_bounds__(kCheckBlock) void k_digest(const BITS *bm, uint64_t
 }
 
 void grid_for(uint64_t words, unsigned *blocks, uint64_t *chunk) {
-  const uint64_t want = (words + 8191) / 8192;
+  const uint64_t want = (words + 2047) / 2048;  // 8 words per thread: enough workgroups in flight
   *blocks = (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(kCheckMaxBlocks, want));
   *chunk = (words + *blocks - 1) / *blocks;
   *blocks = (unsigned)((words + *chunk - 1) / *chunk);
@@ -129,7 +195,8 @@ struct CheckScratch {
   void *host = nullptr;
 };
 
-constexpr size_t kCheckScratchBytes = kCheckMaxBlocks * sizeof(ComparePart);
+// the parts of one check, then (compare) its folded result
+constexpr size_t kCheckScratchBytes = kCheckMaxBlocks * sizeof(ComparePart) + sizeof(rts_compare_result);
 
 hipError_t check_scratch(hipStream_t stream, CheckScratch *out) {
   static std::mutex mu;
@@ -168,47 +235,19 @@ hipError_t bitmap_compare(const BITS *truth, const BITS *got, int64_t n, rts_com
   hipError_t e = check_scratch(s, &sc);
   if (e != hipSuccess) return e;
   ComparePart *d_parts = static_cast<ComparePart *>(sc.dev);
+  rts_compare_result *d_res = reinterpret_cast<rts_compare_result *>(d_parts + kCheckMaxBlocks);
   hipLaunchKernelGGL(k_compare, dim3(blocks), dim3(kCheckBlock), 0, s, truth, got, words, tail_mask((uint64_t)n),
                      chunk, d_parts);
   e = hipGetLastError();
-  if (e == hipSuccess)
-    e = hipMemcpyAsync(sc.host, d_parts, blocks * sizeof(ComparePart), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_compare_finalize, dim3(1), dim3(kFinalThreads), 0, s, d_parts, blocks, truth, got, words,
+                       tail_mask((uint64_t)n), d_res);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(sc.host, d_res, sizeof(rts_compare_result), hipMemcpyDeviceToHost, s);
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   if (e != hipSuccess) return e;
-  const std::vector<ComparePart> parts(static_cast<const ComparePart *>(sc.host),
-                                       static_cast<const ComparePart *>(sc.host) + blocks);
-  rts_compare_result r{0, 0, -1, 0, 0};
-  bool stopped = false;
-  for (const ComparePart &p : parts) {
-    r.truth_count += p.truth;
-    r.got_count += p.got;
-    if (stopped) continue;
-    if (p.first_word == kNone) {
-      r.total_hit += p.truth;
-      r.true_hit += p.both;
-    } else {
-      r.total_hit += p.truth_upto;
-      r.true_hit += p.both_upto;
-      stopped = true;
-    }
-  }
-  if (stopped) {  // the first differing row: the leading differing bit of the first word
-    for (const ComparePart &p : parts) {
-      if (p.first_word == kNone) continue;
-      uint32_t a = 0, b = 0;
-      e = hipMemcpy(&a, truth + p.first_word, 4, hipMemcpyDeviceToHost);
-      if (e == hipSuccess) e = hipMemcpy(&b, got + p.first_word, 4, hipMemcpyDeviceToHost);
-      if (e != hipSuccess) return e;
-      const uint32_t last = tail_mask((uint64_t)n);
-      if (p.first_word == words - 1) {
-        a &= last;
-        b &= last;
-      }
-      r.first_mismatch = (long long)(p.first_word * 32 + (uint64_t)__builtin_clz(a ^ b));
-      break;
-    }
-  }
-  *res = r;
+  *res = *static_cast<const rts_compare_result *>(sc.host);
   return hipSuccess;
 }
 
