@@ -176,3 +176,20 @@ def test_entry_points_reject_bad_arguments_before_any_device_work():
     assert L.rts_encode_column(None, 10, 0, 10, None, None, None, None, None, ctypes.byref(m),
                                None) == HIP_ERROR_INVALID_VALUE
     assert L.rts_scan_range(None, 10, 0, 5, None, None, None) == HIP_ERROR_INVALID_VALUE
+
+
+def test_batch_default_config_is_the_documented_shape():
+    """The default launch shape (include/rts_batch.h, used when a plan gets no config):
+    2048-word tiles, up to 32 queries per work item, 4 workgroups per CU, locality order,
+    256-thread workgroups; shapes outside the kernels' instantiations are refused."""
+    cfg = R.BatchConfig()
+    R.lib().rts_batch_default_config(ctypes.byref(cfg))
+    assert cfg.astuple() == (2048, 32, 4, 1, 256)
+    L = R.lib()
+    plan = ctypes.c_void_p()
+    cols = (ctypes.c_void_p * 1)(None)
+    # invalid shapes are refused before any device work (valid ones would allocate)
+    for bad in ((2048, 32, 4, 1, 128), (4096, 32, 4, 1, 256), (2048, 65, 4, 1, 256)):
+        c = R.BatchConfig(*bad)
+        assert L.rts_batch_plan_create(ctypes.byref(plan), cols, None, 1, 1000, 4, ctypes.byref(c),
+                                       None) == HIP_ERROR_INVALID_VALUE, bad
