@@ -7,8 +7,9 @@
  * and row positions, K-1 filter vectors in one allocation, per-bin refine lists in row
  * order) behind `dev`.  The fields the reference's callers read keep their names and
  * meaning: length, areaStartValues, area_counts, data_min, data_max, filterVectorsInGPU,
- * rawDataInGPU, and areas[k] (block counts and block boundary values, host side, enough
- * for in_which_block).
+ * rawDataInGPU, areas[k] and areasInGPU[k] with their blocks[b]->pos / val / length
+ * (views into the flat sorted arrays: host copies for areas, device memory for
+ * areasInGPU), plus per-block boundary values for in_which_block.
  */
 #ifndef RTS_BINDEX_H_
 #define RTS_BINDEX_H_
@@ -22,12 +23,26 @@ extern "C" {
 #define RTS_VAREA_N 128          /* K, bindex.h:35-37 */
 #define RTS_BLOCK_INIT_SIZE 3276 /* blockInitSize, bindex.h:104 */
 
-/* Area, bindex.h:116-120: blockNum blocks of RTS_BLOCK_INIT_SIZE rows (the last holds the
- * rest) over the area's slice [start, start + length) of the value-sorted column. */
+/* pos_block, bindex.h:112-117: one block of an area's value-sorted rows.  Here a view
+ * into the column's flat value-sorted arrays (ties in row order). */
+typedef struct pos_block {
+  POSTYPE *pos; /* row positions of the block's rows, in value order */
+  CODE *val;    /* their values, ascending */
+  int length;
+} pos_block;
+
+/* Area, bindex.h:118-122: blockNum blocks of RTS_BLOCK_INIT_SIZE rows (the last holds the
+ * rest) over the area's slice [start, start + length) of the value-sorted column.  The
+ * reference's fixed array blocks[blockNumMax] (blockNumMax fixed by the compile-time row
+ * count) is a pointer to blockNum block pointers here: area->blocks[b]->pos reads the same.
+ * In BinDex::areas the blocks hold host pointers and exist once rts_bindex_host_blocks has
+ * run (init_bindex_in_GPU runs it; blocks is NULL before); in BinDex::areasInGPU they
+ * hold device pointers and always exist. */
 typedef struct Area {
+  pos_block **blocks;
   int blockNum;
   int length;
-  POSTYPE start;
+  POSTYPE start;    /* first sorted slot of the area */
   CODE *blockFirst; /* host, blockNum: smallest value of each block */
   CODE *blockLast;  /* host, blockNum: largest value of each block */
 } Area;
@@ -44,7 +59,7 @@ typedef struct BinDex {
   POSTYPE length;
   CODE data_min;
   CODE data_max;
-  Area *areasInGPU[RTS_VAREA_N]; /* always NULL: replaced by the flat device index */
+  Area *areasInGPU[RTS_VAREA_N]; /* blocks[b]->pos / val point into device memory */
   CODE *rawDataInGPU;            /* device column (owned by the caller or by the library) */
   struct rts_bindex_dev *dev;    /* engine-owned device index */
 } BinDex;
@@ -75,6 +90,10 @@ hipError_t GPURefineEqAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CO
 int in_which_area(BinDex *bindex, CODE compare);
 /* in_which_block (bindex.h:183, bindex.cpp:1153-1176) */
 int in_which_block(Area *area, CODE compare);
+/* on_which_pos (bindex.cpp:1201-1215, bindex_cuda.cpp:756-770): index of the first value
+ * >= compare in a host block (pb->length when every value is below it).  The reference
+ * asserts compare >= pb->val[0]; this returns 0 there. */
+int on_which_pos(pos_block *pb, CODE compare);
 
 /* ------------------------------------------------------------------ additive BinDex API */
 
@@ -84,6 +103,10 @@ int in_which_block(Area *area, CODE compare);
 hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, hipStream_t stream);
 /* As rts_bindex_build for a host column, uploaded into a library-owned device copy. */
 hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n);
+/* Host copies of the value-sorted values and positions behind BinDex::areas[k]->blocks
+ * (8 bytes per row of host memory), as the reference keeps them (bindex.cpp:300-309).
+ * Idempotent.  Synchronous. */
+hipError_t rts_bindex_host_blocks(BinDex *bindex);
 /* Release the device index (and the library-owned column copy, if any). */
 void rts_bindex_free(BinDex *bindex);
 /* Index persistence (no reference counterpart; the reference rebuilds at every start,
@@ -116,9 +139,11 @@ long long rts_bindex_device_bytes(const BinDex *bindex);
 
 #ifdef __cplusplus
 /* init_bindex_in_GPU (bindex_cuda.cpp:164-282): the reference driver's build call, with its
- * signature.  base_data grows by the device bytes of the uploaded column, as there. */
+ * signature.  base_data grows by the device bytes of the uploaded column, as there; the
+ * areas' host blocks are filled, as the reference's host build leaves them. */
 inline void init_bindex_in_GPU(BinDex *bindex, CODE *data, POSTYPE n, size_t &base_data) {
-  if (rts_bindex_build_host(bindex, data, n) == hipSuccess) base_data += (size_t)n * sizeof(CODE);
+  if (rts_bindex_build_host(bindex, data, n) == hipSuccess && rts_bindex_host_blocks(bindex) == hipSuccess)
+    base_data += (size_t)n * sizeof(CODE);
 }
 #endif
 #endif /* RTS_BINDEX_H_ */

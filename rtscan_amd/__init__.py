@@ -76,6 +76,8 @@ SIGNATURES = {
     "GPURefineEqAreaWithCuda": (_err, [_vp, _vp, _vp, _int, _int, ctypes.c_bool]),
     "in_which_area": (_int, [_vp, _u32]),
     "in_which_block": (_int, [_vp, _u32]),
+    "on_which_pos": (_int, [_vp, _u32]),
+    "rts_bindex_host_blocks": (_err, [_vp]),
     "rts_bindex_build": (_err, [_vp, _vp, _int, _vp]),
     "rts_bindex_build_host": (_err, [_vp, _vp, _int]),
     "rts_bindex_free": (None, [_vp]),
@@ -421,6 +423,10 @@ class BinDexIndex:
             check(lib().rts_bindex_load(ctypes.byref(self.s), os.fsencode(snapshot),
                                         column.ptr if column is not None else None),
                   "rts_bindex_load")
+            if self.s.length != n:
+                lib().rts_bindex_free(ctypes.byref(self.s))
+                raise ValueError("snapshot %r indexes %d rows, the column has %d"
+                                 % (snapshot, self.s.length, n))
         else:
             check(lib().rts_bindex_build(ctypes.byref(self.s), column.ptr, n, _sh(stream)),
                   "rts_bindex_build")
@@ -462,6 +468,27 @@ class BinDexIndex:
         check(lib().rts_bindex_export_sorted(ctypes.byref(self.s), v.ctypes.data, p.ctypes.data),
               "export_sorted")
         return v, p
+
+    def host_blocks(self):
+        """Fill the areas' host pos_block views (areas[k]->blocks[b]->pos / val)."""
+        check(lib().rts_bindex_host_blocks(ctypes.byref(self.s)), "rts_bindex_host_blocks")
+
+    def block(self, area, b, gpu=False):
+        """(pos, val) of block b of an area as numpy arrays: host views (after host_blocks())
+        or, with gpu=True, copied from the device views of areasInGPU."""
+        a = (self.s.areasInGPU if gpu else self.s.areas)[area].contents
+        if not a.blocks:
+            raise ValueError("host blocks not filled: call host_blocks()")
+        pb = a.blocks[b].contents
+        if gpu:
+            pos = np.empty(pb.length, np.int32)
+            val = np.empty(pb.length, np.uint32)
+            if pb.length:
+                check(lib().rts_memcpy_dtoh(pos.ctypes.data, ctypes.cast(pb.pos, _vp), 4 * pb.length), "dtoh")
+                check(lib().rts_memcpy_dtoh(val.ctypes.data, ctypes.cast(pb.val, _vp), 4 * pb.length), "dtoh")
+            return pos, val
+        return (np.ctypeslib.as_array(pb.pos, (pb.length,)).copy(),
+                np.ctypeslib.as_array(pb.val, (pb.length,)).copy())
 
     def refine_entries(self):
         return int(lib().rts_bindex_refine_entries(ctypes.byref(self.s)))

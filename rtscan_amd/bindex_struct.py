@@ -5,8 +5,17 @@ VAREA_N = 128  # RTS_VAREA_N, K of the reference (bindex.h:35-37)
 BLOCK_INIT_SIZE = 3276  # RTS_BLOCK_INIT_SIZE (bindex.h:104)
 
 
+class PosBlock(ctypes.Structure):
+    _fields_ = [
+        ("pos", ctypes.POINTER(ctypes.c_int32)),
+        ("val", ctypes.POINTER(ctypes.c_uint32)),
+        ("length", ctypes.c_int),
+    ]
+
+
 class Area(ctypes.Structure):
     _fields_ = [
+        ("blocks", ctypes.POINTER(ctypes.POINTER(PosBlock))),
         ("blockNum", ctypes.c_int),
         ("length", ctypes.c_int),
         ("start", ctypes.c_int),
@@ -25,7 +34,7 @@ class BinDex(ctypes.Structure):
         ("length", ctypes.c_int),
         ("data_min", ctypes.c_uint32),
         ("data_max", ctypes.c_uint32),
-        ("areasInGPU", ctypes.c_void_p * VAREA_N),
+        ("areasInGPU", ctypes.POINTER(Area) * VAREA_N),
         ("rawDataInGPU", ctypes.c_void_p),
         ("dev", ctypes.c_void_p),
     ]

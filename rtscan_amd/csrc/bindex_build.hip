@@ -173,10 +173,31 @@ static void free_dev(rts_bindex_dev *d) {
   delete d;
 }
 
+// pos_block views of every block over the sorted arrays `val` / `pos` (device or host):
+// block b of area k covers slots [start_k + b * 3276, ...) (init_pos_block,
+// bindex.cpp:300-309).
+static void make_block_views(const rts_bindex_dev *d, CODE *val, int32_t *pos, std::vector<pos_block> &blk,
+                             std::vector<pos_block *> &ptrs) {
+  blk.clear();
+  for (int k = 0; k < kBins; k++) {
+    const int64_t len = d->area_len[k];
+    const int nb = (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
+    for (int b = 0; b < nb; b++) {
+      const int64_t s0 = d->area_start[k] + (int64_t)b * RTS_BLOCK_INIT_SIZE;
+      const int64_t s1 = std::min<int64_t>(s0 + RTS_BLOCK_INIT_SIZE, d->area_start[k] + len);
+      blk.push_back(pos_block{(POSTYPE *)(pos + s0), val + s0, (int)(s1 - s0)});
+    }
+  }
+  ptrs.resize(blk.size());
+  for (size_t i = 0; i < blk.size(); i++) ptrs[i] = &blk[i];
+}
+
 // The reference-shaped descriptor (bindex.h:126-138) over a complete device index: areas
 // and their blocks (bindex.cpp:633-652), start values, counts, filter-vector pointers.
 void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
   d->area_store.resize(kBins);
+  d->area_gpu_store.resize(kBins);
+  make_block_views(d, d->sorted_val, d->sorted_pos, d->blk_dev, d->blkp_dev);
   int64_t b0 = 0;
   for (int k = 0; k < kBins; k++) {
     Area &a = d->area_store[k];
@@ -186,6 +207,10 @@ void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
     a.start = (POSTYPE)d->area_start[k];
     a.blockFirst = d->block_first.data() + b0;
     a.blockLast = d->block_last.data() + b0;
+    a.blocks = d->blkp_host.empty() ? nullptr : d->blkp_host.data() + b0;
+    Area &g = d->area_gpu_store[k];
+    g = a;
+    g.blocks = d->blkp_dev.data() + b0;
     b0 += a.blockNum;
   }
   bx->length = (POSTYPE)d->n;
@@ -194,7 +219,7 @@ void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
     bx->areas[k] = &d->area_store[k];
     bx->areaStartValues[k] = d->starts[k];
     bx->area_counts[k] = (POSTYPE)(d->area_start[k] + d->area_len[k]);
-    bx->areasInGPU[k] = nullptr;
+    bx->areasInGPU[k] = &d->area_gpu_store[k];
   }
   for (int k = 0; k < kBins - 1; k++) {
     bx->filterVectors[k] = nullptr;
@@ -202,6 +227,30 @@ void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
   }
   bx->rawDataInGPU = const_cast<CODE *>(col);
   bx->dev = d;
+}
+
+// Host copies of the sorted arrays and the host pos_block views over them.
+hipError_t bindex_host_blocks(BinDex *bx) {
+  if (!bx || !bx->dev) return hipErrorInvalidValue;
+  rts_bindex_dev *d = bx->dev;
+  if (!d->blkp_host.empty()) return hipSuccess;
+  d->host_val.resize(d->n);
+  d->host_pos.resize(d->n);
+  hipError_t e = hipMemcpy(d->host_val.data(), d->sorted_val, d->n * sizeof(CODE), hipMemcpyDeviceToHost);
+  if (e == hipSuccess)
+    e = hipMemcpy(d->host_pos.data(), d->sorted_pos, d->n * sizeof(int32_t), hipMemcpyDeviceToHost);
+  if (e != hipSuccess) {
+    std::vector<CODE>().swap(d->host_val);
+    std::vector<int32_t>().swap(d->host_pos);
+    return e;
+  }
+  make_block_views(d, d->host_val.data(), d->host_pos.data(), d->blk_host, d->blkp_host);
+  int64_t b0 = 0;
+  for (int k = 0; k < kBins; k++) {
+    d->area_store[k].blocks = d->blkp_host.data() + b0;
+    b0 += d->area_store[k].blockNum;
+  }
+  return hipSuccess;
 }
 
 hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s) {

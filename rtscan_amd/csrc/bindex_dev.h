@@ -51,8 +51,15 @@ struct rts_bindex_dev {
   int64_t area_start[rts::kBins];
   int64_t area_len[rts::kBins];
   std::vector<uint32_t> boff_host;
-  std::vector<Area> area_store;
+  std::vector<Area> area_store;      // BinDex::areas
+  std::vector<Area> area_gpu_store;  // BinDex::areasInGPU
   std::vector<CODE> block_first, block_last;
+  // pos_block views (bindex.h:112-117): device views always, host views once the host
+  // copies of the sorted arrays exist (rts_bindex_host_blocks)
+  std::vector<pos_block> blk_dev, blk_host;
+  std::vector<pos_block *> blkp_dev, blkp_host;
+  std::vector<CODE> host_val;
+  std::vector<int32_t> host_pos;
 };
 
 namespace rts {

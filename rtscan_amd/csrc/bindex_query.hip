@@ -21,6 +21,7 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
 void bindex_free(BinDex *bx);
 hipError_t bindex_save(const BinDex *bx, const char *path);
 hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col);
+hipError_t bindex_host_blocks(BinDex *bx);
 hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
                  hipStream_t s);
 
@@ -337,6 +338,20 @@ int in_which_block(Area *area, CODE compare) {
   return res;
 }
 
+// on_which_pos, bindex.cpp:1201-1215 (dup bindex_cuda.cpp:756-770): lower bound of
+// `compare` in the block's ascending values.
+int on_which_pos(pos_block *pb, CODE compare) {
+  int lo = 0, hi = pb->length;
+  while (lo < hi) {
+    const int mid = (lo + hi) / 2;
+    if (pb->val[mid] >= compare)
+      hi = mid;
+    else
+      lo = mid + 1;
+  }
+  return lo;
+}
+
 // copy_filter_vector_in_GPU, bindex_cuda.cpp:486-504 (declared bindex.h:145)
 void copy_filter_vector_in_GPU(BinDex *bindex, BITS *dev_bitmap, int k, bool negation) {
   if (!bindex || !dev_bitmap) return;
@@ -442,6 +457,10 @@ hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYP
 }
 
 void rts_bindex_free(BinDex *bindex) { bindex_free(bindex); }
+
+hipError_t rts_bindex_host_blocks(BinDex *bindex) {
+  return set_error(bindex_host_blocks(bindex), "rts_bindex_host_blocks");
+}
 
 hipError_t rts_bindex_query(const BinDex *const *bindexes, const CODE *lo, const CODE *hi,
                             int ncols, BITS *out, unsigned long long *count, hipStream_t stream) {

@@ -193,3 +193,22 @@ def test_batch_default_config_is_the_documented_shape():
         c = R.BatchConfig(*bad)
         assert L.rts_batch_plan_create(ctypes.byref(plan), cols, None, 1, 1000, 4, ctypes.byref(c),
                                        None) == HIP_ERROR_INVALID_VALUE, bad
+
+
+def test_reference_shaped_caller_compiles_and_runs_on_host(tmp_path):
+    """A translation unit shaped like the reference's drivers (bindex_cuda.cpp:723-770,
+    bindex.cpp:1365-1411: area->blocks[b]->pos / val / length, in_which_area,
+    in_which_block, on_which_pos) compiles against include/rts_bindex.h unchanged, links to
+    the engine, and its host-only path (a descriptor built by hand, the library's lookups)
+    answers `x < c` exactly."""
+    import subprocess
+    exe = str(tmp_path / "ref_shaped_lt")
+    src = os.path.join(ROOT, "tests", "ref_caller", "ref_shaped_lt.cpp")
+    libdir = os.path.dirname(R.LIB_PATH)
+    subprocess.run(["g++", "-O1", "-std=c++17", "-Wall", "-Werror", "-D__HIP_PLATFORM_AMD__",
+                    "-I/opt/rocm/include", "-I" + os.path.join(ROOT, "include"), "-o", exe, src,
+                    "-L" + libdir, "-lrtscan_hip", "-L/opt/rocm/lib", "-lamdhip64",
+                    "-Wl,-rpath," + libdir, "-Wl,-rpath,/opt/rocm/lib"], check=True)
+    r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stderr
+    assert r.stdout.startswith("ok ")

@@ -1,6 +1,8 @@
 """GPU parity of the BinDex path (kernel b, fused with the multi-column AND of kernel c) and
 of the reference's BinDex call surface, through the C ABI, against the CPU oracle."""
 import ctypes
+import os
+import subprocess
 
 import numpy as np
 import pytest
@@ -9,6 +11,7 @@ import oracle as O
 from _util import MAXU, closed_form, interval_mask, pack_rows
 
 pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def upload(eng, arr):
@@ -233,3 +236,50 @@ def test_scale_1e8_bindex(eng):
             O.raw_scan(x, O.LE, hi, threads=16)
         assert np.array_equal(out.to_numpy(np.uint32, n // 32), truth)
         assert int(cnt.to_numpy(np.uint64, 1)[0]) == O.popcount_rows(truth, n)
+
+
+@pytest.mark.parametrize("n", [1000, 1 << 20])
+def test_pos_block_views_match_oracle(eng, n):
+    """areas[k]->blocks[b] (host views, after host_blocks()) and areasInGPU[k]->blocks[b]
+    (device views) hold the oracle's sorted positions and values block by block, and
+    on_which_pos over the engine's blocks equals the oracle's (bindex.cpp:1201-1215)."""
+    L = eng.lib()
+    for name, x in datasets(eng, n).items():
+        dx = upload(eng, x)
+        ix = eng.BinDexIndex(dx, n)
+        ob = O.Bindex(x)
+        assert not ix.s.areas[0].contents.blocks  # no host copies until asked for
+        ix.host_blocks()
+        ix.host_blocks()  # idempotent
+        nums = ob.block_nums()
+        starts = ob.area_start()
+        for k in list(range(0, 128, 9)) + [127]:
+            a = ix.s.areas[k].contents
+            assert a.blockNum == nums[k], (name, k)
+            slot = int(starts[k])
+            for b in range(a.blockNum):
+                pos, val = ix.block(k, b)
+                want_len = min(3276, int(a.length) - 3276 * b)
+                assert len(pos) == want_len, (name, k, b)
+                assert np.array_equal(val, ob.sorted_val[slot:slot + want_len]), (name, k, b)
+                assert np.array_equal(pos, ob.sorted_pos[slot:slot + want_len]), (name, k, b)
+                if b in (0, a.blockNum - 1):
+                    gpos, gval = ix.block(k, b, gpu=True)
+                    assert np.array_equal(gpos, pos) and np.array_equal(gval, val)
+                for c in {int(val[0]), int(val[-1]), int(val[len(val) // 2]) + 1}:
+                    pb = ctypes.cast(a.blocks[b], ctypes.c_void_p).value
+                    assert L.on_which_pos(pb, c) == ob.on_which_pos(k, b, c), (name, k, b, c)
+                slot += want_len
+        ix.free()
+
+
+def test_reference_shaped_caller_runs_on_the_engine(eng):
+    """A caller written like the reference's BinDex drivers (area->blocks[b]->pos,
+    in_which_block, on_which_pos, copy_filter_vector_in_GPU, host refine), compiled
+    against include/ and linked to the engine, answers `x < c` exactly."""
+    exe = os.path.join(ROOT, "tests", "ref_caller", "ref_shaped_lt")
+    assert os.path.isfile(exe), "build the test caller first (make -C tests/ref_caller)"
+    print("ref_shaped_lt gpu 1000003", flush=True)
+    r = subprocess.run([exe, "gpu", "1000003"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert r.stdout.startswith("ok ")
