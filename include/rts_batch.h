@@ -4,8 +4,9 @@
  *
  * A query is an AND of closed ranges over 1..RTS_BATCH_MAX_COLS columns of a table whose
  * columns all have n rows.  Each query-column is evaluated from the column's BinDex (filter
- * vectors + boundary-bin refine lists) or directly from its values; RTS_EVAL_AUTO takes the
- * BinDex when the column has one.  Query q's bitmap is written at out + q * out_stride_words
+ * vectors + boundary-bin refine lists) or directly from its values; for RTS_EVAL_AUTO the
+ * planner's cost model chooses (the bytes per row each form reads, the direct form weighted
+ * by the config's direct_weight; a column without an index is evaluated directly).  Query q's bitmap is written at out + q * out_stride_words
  * (MSB-first words, bits past row n cleared) and its popcount at counts[q] (device).
  *
  * The launch is one persistent kernel over (row tile, query group) work items, handed out by
@@ -49,6 +50,11 @@ typedef struct rts_batch_config {
   int order;
   int threads;  /* workgroup size: 512 (64 KB of LDS toggles) or 256 (32 KB, half the
                    column slots per work item, twice the work items per CU) */
+  int bins;     /* engine bins of the index set the queries use (0: the set the plan was
+                   created with; see rts_batch_plan_add_indexes) */
+  int direct_weight; /* cost of evaluating an RTS_EVAL_AUTO query-column directly, per mille
+                        of its column's 4 bytes per row, against the BinDex form's filter-
+                        vector and list bytes per row; 0: never direct when indexed */
 } rts_batch_config;
 
 typedef struct rts_batch_plan rts_batch_plan;
@@ -61,10 +67,16 @@ void rts_batch_default_config(rts_batch_config *cfg);
 hipError_t rts_batch_plan_create(rts_batch_plan **plan, const CODE *const *columns,
                                  const BinDex *const *indexes, int ncols, int64_t n,
                                  int max_queries, const rts_batch_config *cfg, hipStream_t stream);
-/* Change the launch shape (a new tile size re-tiles the list directories and drops the
- * current queries: set them again). */
+/* Change the launch shape.  The current queries are planned again for it (a new tile size
+ * or index set also rebuilds the list directories); a config naming bins no index set of
+ * the plan has is refused. */
 hipError_t rts_batch_plan_set_config(rts_batch_plan *plan, const rts_batch_config *cfg,
                                      hipStream_t stream);
+/* Give the plan another index set of the same table: the same columns indexed with another
+ * bin count (rts_bindex_build_bins; entries may be NULL), selected by a config whose bins
+ * is that count.  A set with the bin count of an existing inactive set replaces it.
+ * *bins (may be NULL) receives the set's bin count.  The indexes must outlive the plan. */
+hipError_t rts_batch_plan_add_indexes(rts_batch_plan *plan, const BinDex *const *indexes, int *bins);
 /* Plan nq <= max_queries queries (host array) and upload them; returns after the upload. */
 hipError_t rts_batch_plan_set_queries(rts_batch_plan *plan, const rts_query_spec *queries,
                                       int nq, hipStream_t stream);
@@ -85,14 +97,27 @@ hipError_t rts_batch_plan_graph_launch(rts_batch_plan *plan, BITS *out, int64_t 
  * path, BinDex columns whose range lies inside one bin (no filter vector read); row tiles. */
 void rts_batch_plan_stats(const rts_batch_plan *plan, int *bindex_cols, int *direct_cols,
                           int *narrow_cols, long long *ntiles);
+/* RTS_EVAL_AUTO query-columns of the current queries that the cost model sent to the
+ * direct form although their column has an index (-1 for a NULL plan). */
+int rts_batch_plan_auto_direct(const rts_batch_plan *plan);
+/* 64-bit digest of the current query mix: a histogram of query shapes (column count, and
+ * per column its evaluation mode and a log2 bucket of the fraction of the column's 128 areas
+ * the range touches), each cell's share rounded to 1/64.  Batches drawn from one distribution get
+ * the same digest; the autotuner's cache is keyed by it. */
+unsigned long long rts_batch_plan_mix_digest(const rts_batch_plan *plan);
+/* Work-item groups the current queries were planned into, and (cfg non-NULL) the launch
+ * shape they were planned for; -1 for a NULL plan. */
+int rts_batch_plan_groups(const rts_batch_plan *plan, rts_batch_config *cfg);
 /* Rows, table columns and current query count of a plan. */
 void rts_batch_plan_info(const rts_batch_plan *plan, long long *n, int *ncols, int *nq);
 void rts_batch_plan_free(rts_batch_plan *plan);
 
 /* Autotuner: time every candidate launch shape on the plan's current queries (median of
- * `reps` launches each) and keep the fastest in the plan.  With cache_path non-NULL the
- * result is read from / appended to that file, keyed by device, n, column count and query
- * count, so later runs skip the search.  best_ms (may be NULL) receives the winner's time. */
+ * `reps` launches each) and keep the fastest in the plan.  Candidates span tile size, group
+ * size, workgroups per CU, order, workgroup size, bin count (the plan's index sets) and the
+ * planner's direct weight.  With cache_path non-NULL the result is read from / appended to
+ * that file, keyed by device, n, column count, query count and the query-mix digest, so
+ * later runs skip the search.  best_ms (may be NULL) receives the winner's time. */
 hipError_t rts_batch_autotune(rts_batch_plan *plan, BITS *out, int64_t out_stride_words,
                               const rts_batch_config *candidates, int ncandidates, int reps,
                               const char *cache_path, rts_batch_config *chosen, float *best_ms,

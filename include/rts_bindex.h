@@ -101,6 +101,16 @@ int on_which_pos(pos_block *pb, CODE compare);
  * column is referenced, not copied (BinDex::rawDataInGPU), and must outlive the index.
  * n >= RTS_VAREA_N.  Synchronous. */
 hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, hipStream_t stream);
+/* As rts_bindex_build with `bins` engine bins (128, 256, 512 or 1024): each of the
+ * reference's 128 areas is cut into bins / 128 equal-count bins, each with its filter
+ * vector and refine list, so a query bound refines a list bins / 128 times shorter for
+ * (bins - 1) * n / 8 bytes of filter vectors.  The reference-facing fields (areas, area
+ * starts, filterVectorsInGPU[k] = x < areaStartValues[k + 1]) are the same for every bin
+ * count; rts_bindex_query and the batched launch use the engine bins. */
+hipError_t rts_bindex_build_bins(BinDex *bindex, const CODE *dev_column, POSTYPE n, int bins,
+                                 hipStream_t stream);
+/* Engine bins of a built index (-1 if none). */
+int rts_bindex_bins(const BinDex *bindex);
 /* As rts_bindex_build for a host column, uploaded into a library-owned device copy. */
 hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYPE n);
 /* Host copies of the value-sorted values and positions behind BinDex::areas[k]->blocks
@@ -113,8 +123,9 @@ void rts_bindex_free(BinDex *bindex);
  * bindex_cuda.cpp:1860-1890): write a built index to a binary snapshot file, and load one
  * into device memory without sorting or rebuilding the filter vectors.  `dev_column` is the
  * indexed column already on the device (kept as rawDataInGPU for the refine surface; may
- * be NULL).  A snapshot whose header does not match (format, bin count, sizes) is refused
- * with hipErrorInvalidValue.  Synchronous. */
+ * be NULL).  A snapshot whose header, host arrays, refine-list rows or per-array digests
+ * do not check out is refused with hipErrorInvalidValue before the index can be used.
+ * Synchronous. */
 hipError_t rts_bindex_save(const BinDex *bindex, const char *path);
 hipError_t rts_bindex_load(BinDex *bindex, const char *path, const CODE *dev_column);
 

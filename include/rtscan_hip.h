@@ -119,8 +119,15 @@ hipError_t rts_event_elapsed_ms(float *ms, hipEvent_t start, hipEvent_t stop);
 /* Device count, name and compute-unit count of the current device. */
 int rts_device_count(void);
 hipError_t rts_device_info(char *name, size_t name_len, int *cu_count, size_t *total_mem);
+/* Free and total device memory of the current device (hipMemGetInfo). */
+hipError_t rts_device_mem_info(size_t *free_bytes, size_t *total_bytes);
 /* Message of the last error recorded by this library on the calling thread. */
 const char *rts_last_error(void);
+/* Scratch buffers (count partials, checker parts) the calling thread holds: one per
+ * (kind, device, stream) it has used, at most 16 (the oldest is freed first); a thread's
+ * buffers are freed when it exits.  Counted and checking calls allocate on their first use
+ * of a stream, so that first call is not for graph capture. */
+int rts_thread_scratch_buffers(void);
 
 /* ------------------------------------------------------------ additive: scan kernels */
 

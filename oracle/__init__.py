@@ -88,6 +88,9 @@ def lib():
         L.orc_bindex_scan.argtypes = [ctypes.POINTER(_OrcBindex), _u32p, ctypes.c_int,
                                       ctypes.c_uint32, ctypes.c_uint32, ctypes.c_int]
         L.orc_bitmap_digest.argtypes = [_u32p, ctypes.c_int64]
+        L.orc_conjunction_digest.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _u32p, _u32p,
+                                             ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
+        L.orc_conjunction_digest.restype = None
         L.orc_bitmap_digest.restype = ctypes.c_uint64
         L.orc_normal_encode.argtypes = [_u32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                         _u32p, _i32p, _u32p, _u32p, _u32p]
@@ -155,6 +158,21 @@ def conjunction(columns, preds, threads=8):
     for c, (col, (op, tl, tr)) in enumerate(zip(columns, preds)):
         raw_scan_entry(col, op, tl, tr, bitmap=bms[c], merge=bms[0], threads=threads)
     return bms[0]
+
+
+def conjunction_digest(columns, ranges, threads=None):
+    """(digest, popcount) of the AND over `columns` of closed ranges [(lo, hi), ...] (lo > hi:
+    empty), computed without the bitmap (orc_conjunction_digest)."""
+    cols = [np.ascontiguousarray(c, dtype=np.uint32) for c in columns]
+    n = cols[0].size
+    assert all(c.size == n for c in cols) and len(ranges) == len(cols)
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    lo = np.array([r[0] & 0xFFFFFFFF for r in ranges], np.uint32)
+    hi = np.array([r[1] & 0xFFFFFFFF for r in ranges], np.uint32)
+    d, c = ctypes.c_uint64(), ctypes.c_uint64()
+    lib().orc_conjunction_digest(ptrs, len(cols), n, lo, hi, threads or min(16, os.cpu_count() or 1),
+                                 ctypes.byref(d), ctypes.byref(c))
+    return d.value, c.value
 
 
 def compare_bitmap(truth, got, n):

@@ -688,6 +688,34 @@ static uint64_t mix64(uint64_t x) {
   return x ^ (x >> 31);
 }
 
+/* The checker's conjunction of one multi-column query (bindex.cpp:2265-2289: raw_scan per
+ * column into a zeroed bitmap, AND-merged), for closed ranges lo[c] <= x <= hi[c] (the
+ * raw_scan predicates le / ge / bt of the query's targets), reduced to its popcount and
+ * digest word by word so that a 1e9-row query needs no 125 MB bitmap.  The digest is
+ * orc_bitmap_digest's. */
+void orc_conjunction_digest(const uint32_t *const *cols, int ncols, int64_t n, const uint32_t *lo,
+                            const uint32_t *hi, int threads, uint64_t *digest, uint64_t *count) {
+  uint64_t d = 0, c = 0;
+  const int64_t words = n <= 0 ? 0 : (n + 31) / 32;
+#pragma omp parallel for num_threads(threads) schedule(static, 4096) reduction(+ : d, c)
+  for (int64_t w = 0; w < words; w++) {
+    const int64_t r0 = w * 32;
+    const int rows = n - r0 < 32 ? (int)(n - r0) : 32;
+    uint32_t acc = rows == 32 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> rows);
+    for (int k = 0; k < ncols && acc; k++) {
+      const uint32_t *x = cols[k] + r0;
+      uint32_t m = 0;
+      for (int i = 0; i < rows; i++)
+        if (x[i] >= lo[k] && x[i] <= hi[k]) m |= 0x80000000u >> i;
+      acc &= m;
+    }
+    d += mix64(((uint64_t)w << 32) | acc);
+    c += (uint64_t)__builtin_popcount(acc);
+  }
+  *digest = d;
+  *count = c;
+}
+
 uint64_t orc_bitmap_digest(const uint32_t *bitmap, int64_t n) {
   if (n <= 0) return 0;
   const int64_t words = (n + 31) / 32;

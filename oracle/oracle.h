@@ -19,6 +19,9 @@
  *                                 closed form fv[k] = (x < start[k+1]) in numpy.
  *   orc_in_which_area/_block,     compiled reference on the same BinDex.
  *   orc_on_which_pos
+ *   orc_conjunction_digest        orc_raw_scan_entry (bt / le / ge per column, AND merge) and
+ *                                 orc_bitmap_digest on the same inputs, including empty
+ *                                 ranges and ragged row counts.
  *   orc_bindex_scan               compiled reference bindex_scan_{lt,le,gt,ge,bt,eq}
  *                                 bit-exact on every tested query (including the inputs
  *                                 where the reference disagrees with its own raw_scan),
@@ -95,6 +98,13 @@ int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws
  * mix64((i << 32) | word_i), bits past row n cleared, mix64(x) = splitmix64 finaliser of
  * x ^ 0x9E3779B97F4A7C15.  Not a reference function: a checksum shared by both sides. */
 uint64_t orc_bitmap_digest(const uint32_t *bitmap, int64_t n);
+
+/* Digest (orc_bitmap_digest) and popcount of the AND over ncols columns of n rows of the
+ * closed ranges lo[c] <= x <= hi[c] (lo > hi: no row), computed word by word without the
+ * bitmap (bindex.cpp:2265-2289's checker, reduced).  Pinned against orc_raw_scan_entry +
+ * orc_bitmap_digest on small n. */
+void orc_conjunction_digest(const uint32_t *const *cols, int ncols, int64_t n, const uint32_t *lo,
+                            const uint32_t *hi, int threads, uint64_t *digest, uint64_t *count);
 
 /* normalEncode, 7-argument form (remap.cpp:110-203), for one column: rank-order codes
  * code(row at sorted index i) = encode_min + i * interval (uint32 wrap), interval =

@@ -62,6 +62,8 @@ SIGNATURES = {
     "rts_device_info": (_err, [ctypes.c_char_p, ctypes.c_size_t, ctypes.POINTER(_int),
                                ctypes.POINTER(ctypes.c_size_t)]),
     "rts_last_error": (ctypes.c_char_p, []),
+    "rts_thread_scratch_buffers": (_int, []),
+    "rts_device_mem_info": (_err, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     # scans
     "rts_predicate_interval": (_int, [_int, _u32, _u32, ctypes.POINTER(_u32),
                                       ctypes.POINTER(_u32)]),
@@ -80,6 +82,8 @@ SIGNATURES = {
     "rts_bindex_host_blocks": (_err, [_vp]),
     "rts_bindex_build": (_err, [_vp, _vp, _int, _vp]),
     "rts_bindex_build_host": (_err, [_vp, _vp, _int]),
+    "rts_bindex_build_bins": (_err, [_vp, _vp, _int, _int, _vp]),
+    "rts_bindex_bins": (_int, [_vp]),
     "rts_bindex_free": (None, [_vp]),
     "rts_bindex_query": (_err, [_vp, _vp, _vp, _int, _vp, _vp, _vp]),
     "rts_bindex_export_sorted": (_err, [_vp, _vp, _vp]),
@@ -111,6 +115,10 @@ SIGNATURES = {
     "rts_batch_plan_info": (None, [_vp, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(_int),
                                    ctypes.POINTER(_int)]),
     "rts_batch_plan_free": (None, [_vp]),
+    "rts_batch_plan_groups": (_int, [_vp, _vp]),
+    "rts_batch_plan_auto_direct": (_int, [_vp]),
+    "rts_batch_plan_mix_digest": (ctypes.c_ulonglong, [_vp]),
+    "rts_batch_plan_add_indexes": (_err, [_vp, _vp, ctypes.POINTER(_int)]),
     "rts_batch_autotune": (_err, [_vp, _vp, ctypes.c_int64, _vp, _int, _int, ctypes.c_char_p, _vp,
                                   ctypes.POINTER(ctypes.c_float), _vp]),
     "rts_batch_plan_set_queries_out": (_err, [_vp, _vp, _vp, _int, _vp]),
@@ -244,10 +252,14 @@ class Stream:
     def synchronize(self):
         check(lib().rts_stream_synchronize(self.handle), "stream_synchronize")
 
+    def destroy(self):
+        if self.handle:
+            check(lib().rts_stream_destroy(self.handle), "stream_destroy")
+            self.handle = None
+
     def __del__(self):
         try:
-            if self.handle:
-                lib().rts_stream_destroy(self.handle)
+            self.destroy()
         except Exception:
             pass
 
@@ -282,6 +294,12 @@ def device_info():
     cus, mem = _int(), ctypes.c_size_t()
     check(lib().rts_device_info(name, 128, ctypes.byref(cus), ctypes.byref(mem)), "device_info")
     return {"arch": name.value.decode(), "cus": cus.value, "total_mem": mem.value}
+
+
+def device_free_bytes():
+    f, t = ctypes.c_size_t(), ctypes.c_size_t()
+    check(lib().rts_device_mem_info(ctypes.byref(f), ctypes.byref(t)), "mem_info")
+    return f.value
 
 
 def _sh(stream):
@@ -412,8 +430,9 @@ def encode_key_of(map_values, map_starts, code):
 class BinDexIndex:
     """A BinDex (include/rts_bindex.h) built on the GPU over a device column."""
 
-    def __init__(self, column, n, stream=None, snapshot=None):
-        """Build over `column` (device, n rows), or load a snapshot written by save()."""
+    def __init__(self, column, n, stream=None, snapshot=None, bins=128):
+        """Build over `column` (device, n rows) with `bins` engine bins (128 * 1, 2, 4 or 8),
+        or load a snapshot written by save()."""
         from .bindex_struct import BinDex
 
         self.column = column  # keep the device column alive
@@ -428,8 +447,8 @@ class BinDexIndex:
                 raise ValueError("snapshot %r indexes %d rows, the column has %d"
                                  % (snapshot, self.s.length, n))
         else:
-            check(lib().rts_bindex_build(ctypes.byref(self.s), column.ptr, n, _sh(stream)),
-                  "rts_bindex_build")
+            check(lib().rts_bindex_build_bins(ctypes.byref(self.s), column.ptr, n, bins, _sh(stream)),
+                  "rts_bindex_build_bins")
         self.built = True
 
     def save(self, path):
@@ -449,6 +468,10 @@ class BinDexIndex:
             self.free()
         except Exception:
             pass
+
+    @property
+    def bins(self):
+        return int(lib().rts_bindex_bins(ctypes.byref(self.s)))
 
     def start_values(self):
         return np.array(self.s.areaStartValues[:], np.uint32)
@@ -572,15 +595,16 @@ class QuerySpec(ctypes.Structure):
 
 class BatchConfig(ctypes.Structure):
     _fields_ = [("tile_words", ctypes.c_int), ("queries_per_group", ctypes.c_int),
-                ("blocks_per_cu", ctypes.c_int), ("order", ctypes.c_int), ("threads", ctypes.c_int)]
+                ("blocks_per_cu", ctypes.c_int), ("order", ctypes.c_int), ("threads", ctypes.c_int),
+                ("bins", ctypes.c_int), ("direct_weight", ctypes.c_int)]
 
     def astuple(self):
         return (self.tile_words, self.queries_per_group, self.blocks_per_cu, self.order,
-                self.threads)
+                self.threads, self.bins, self.direct_weight)
 
 
 def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, order=None,
-                 threads=None):
+                 threads=None, bins=None, direct_weight=None):
     c = BatchConfig()
     lib().rts_batch_default_config(ctypes.byref(c))
     if tile_words is not None:
@@ -593,6 +617,10 @@ def batch_config(tile_words=None, queries_per_group=None, blocks_per_cu=None, or
         c.order = order
     if threads is not None:
         c.threads = threads
+    if bins is not None:
+        c.bins = bins
+    if direct_weight is not None:
+        c.direct_weight = direct_weight
     return c
 
 
@@ -661,7 +689,24 @@ class BatchPlan:
         lib().rts_batch_plan_stats(self.h, ctypes.byref(b), ctypes.byref(d), ctypes.byref(nr),
                                    ctypes.byref(nt))
         return {"bindex_cols": b.value, "direct_cols": d.value, "narrow_cols": nr.value,
-                "tiles": nt.value}
+                "tiles": nt.value, "auto_direct_cols": lib().rts_batch_plan_auto_direct(self.h),
+                "groups": lib().rts_batch_plan_groups(self.h, None)}
+
+    def config(self):
+        c = BatchConfig()
+        lib().rts_batch_plan_groups(self.h, ctypes.byref(c))
+        return c
+
+    def add_indexes(self, indexes):
+        """Another index set of the same table (another bin count); returns its bins."""
+        self.extra_sets = getattr(self, "extra_sets", []) + [indexes]  # keep them alive
+        idx = (_vp * len(indexes))(*[(ix.ptr if ix is not None else None) for ix in indexes])
+        b = _int()
+        check(lib().rts_batch_plan_add_indexes(self.h, idx, ctypes.byref(b)), "rts_batch_plan_add_indexes")
+        return b.value
+
+    def mix_digest(self):
+        return int(lib().rts_batch_plan_mix_digest(self.h))
 
     def free(self):
         if getattr(self, "h", None):

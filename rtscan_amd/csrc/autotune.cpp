@@ -1,5 +1,7 @@
-// autotune.cpp -- the batched launch's autotuner: times candidate launch shapes on the
-// plan's current queries and keeps the fastest; results cached in a text file.
+// autotune.cpp -- the batched launch's autotuner: times candidate launch shapes (tile and
+// group sizes, workgroups, order, bin count, the planner's direct weight) on the plan's
+// current queries and keeps the fastest; results cached in a text file keyed by the
+// device, the table, the query count and the query-mix digest.
 #include <hip/hip_runtime_api.h>
 #include <stdio.h>
 #include <string.h>
@@ -14,7 +16,7 @@
 
 namespace {
 
-std::string cache_key(int64_t n, int table_cols, int nq, int qcols) {
+std::string cache_key(int64_t n, int table_cols, int nq, int qcols, unsigned long long mix) {
   int dev = 0;
   char arch[256] = "unknown";
   int cus = 0;
@@ -28,7 +30,9 @@ std::string cache_key(int64_t n, int table_cols, int nq, int qcols) {
   for (char *c = arch; *c; c++)
     if (*c == ' ' || *c == ':') *c = '_';
   std::ostringstream k;
-  k << arch << "/cu" << cus << "/n" << n << "/c" << table_cols << "/q" << nq << "/qc" << qcols;
+  char mixhex[32];
+  snprintf(mixhex, sizeof(mixhex), "%016llx", mix);
+  k << arch << "/cu" << cus << "/n" << n << "/c" << table_cols << "/q" << nq << "/qc" << qcols << "/mix" << mixhex;
   return k.str();
 }
 
@@ -41,7 +45,8 @@ bool cache_lookup(const char *path, const std::string &key, rts_batch_config *cf
     std::string k;
     rts_batch_config c;
     float m;
-    if (ls >> k >> c.tile_words >> c.queries_per_group >> c.blocks_per_cu >> c.order >> c.threads >> m &&
+    if (ls >> k >> c.tile_words >> c.queries_per_group >> c.blocks_per_cu >> c.order >> c.threads >> c.bins >>
+            c.direct_weight >> m &&
         k == key) {
       *cfg = c;  // the last entry for a key wins
       *ms = m;
@@ -54,8 +59,8 @@ bool cache_lookup(const char *path, const std::string &key, rts_batch_config *cf
 void cache_store(const char *path, const std::string &key, const rts_batch_config &c, float ms) {
   FILE *f = fopen(path, "a");
   if (!f) return;
-  fprintf(f, "%s %d %d %d %d %d %.4f\n", key.c_str(), c.tile_words, c.queries_per_group, c.blocks_per_cu,
-          c.order, c.threads, ms);
+  fprintf(f, "%s %d %d %d %d %d %d %d %.4f\n", key.c_str(), c.tile_words, c.queries_per_group, c.blocks_per_cu,
+          c.order, c.threads, c.bins, c.direct_weight, ms);
   fclose(f);
 }
 
@@ -71,7 +76,7 @@ extern "C" hipError_t rts_batch_autotune(rts_batch_plan *plan, BITS *out, int64_
   long long n = 0;
   int table_cols = 0, nq = 0;
   rts_batch_plan_info(plan, &n, &table_cols, &nq);
-  const std::string key = cache_key(n, table_cols, nq, bcols + dcols);
+  const std::string key = cache_key(n, table_cols, nq, bcols + dcols, rts_batch_plan_mix_digest(plan));
   rts_batch_config best{};
   float best_t = 0;
   if (cache_path && cache_lookup(cache_path, key, &best, &best_t)) {

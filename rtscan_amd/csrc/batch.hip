@@ -25,8 +25,11 @@
 // before the one barrier that precedes the AND phase.
 #include <string.h>
 
+#include <math.h>
+
 #include <algorithm>
 #include <array>
+#include <map>
 #include <vector>
 
 #include "batch.h"
@@ -572,7 +575,7 @@ static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   PlannedBound b;
   if (s.fv < 0) {
     b.base = kBaseZero;
-  } else if (s.fv >= kBins - 1) {
+  } else if (s.fv >= d->bins - 1) {
     b.base = kBaseOnes;
   } else {
     b.base = kBaseFv;
@@ -584,13 +587,37 @@ static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   return b;
 }
 
+// Planner cost model of one query-column, in bytes per row of the column.  BinDex form:
+// the filter vectors it reads (4 bytes per 32 rows each) plus its refine lists' entries
+// (8 bytes each, spread over the column); a narrow column (both bounds in one bin) reads
+// one list and no filter vector.  Direct form: the column's 4 bytes per row, scaled by
+// direct_weight / 1000 (the batched kernel reads a direct column's tile once for all its
+// query-columns from L2 but spends VALU work on every row of every one).  Weight 0: never
+// direct.
+static bool direct_cheaper(const rts_bindex_dev *d, const BoundSpec &sl, const BoundSpec &sh, int direct_weight) {
+  if (direct_weight <= 0) return false;
+  const bool narrow = sl.list >= 0 && sl.list == sh.list;
+  double fvs = 0, entries = 0;
+  if (narrow) {
+    entries = (double)list_entries(d, sl.list);
+  } else {
+    fvs = (sl.fv >= 0 && sl.fv < d->bins - 1) + (sh.fv >= 0 && sh.fv < d->bins - 1);
+    entries = (double)list_entries(d, sl.list) + (double)list_entries(d, sh.list);
+  }
+  const double bindex_cost = fvs / 8.0 + entries * 8.0 / (double)d->n;
+  const double direct_cost = 4.0 * direct_weight / 1000.0;
+  return direct_cost < bindex_cost;
+}
+
 }  // namespace rts
 
 using namespace rts;
 
 struct rts_batch_plan {
-  const BinDex *const *index = nullptr;  // per table column (may hold nullptr: direct only)
-  std::vector<const BinDex *> index_store;
+  const BinDex *const *index = nullptr;  // the active set, per table column (nullptr: direct only)
+  std::vector<std::vector<const BinDex *>> index_sets;  // alternative indexes of the table
+  std::vector<int> set_bins;                             // engine bins of each set
+  int active_set = 0;
   std::vector<const CODE *> columns;
   int ncols = 0;
   int64_t n = 0, words = 0;
@@ -613,7 +640,7 @@ struct rts_batch_plan {
   unsigned long long *graph_counts = nullptr;
   int graph_nq = -1;
   int64_t graph_stride = 0;
-  int stats_bindex_cols = 0, stats_direct_cols = 0, stats_narrow_cols = 0;
+  int stats_bindex_cols = 0, stats_direct_cols = 0, stats_narrow_cols = 0, stats_auto_direct = 0;
   std::vector<rts_query_spec> specs;  // the current queries, re-planned on a new tiling
   std::vector<BITS *> outs;           // their per-query outputs, if given
   bool has_outs = false;
@@ -621,31 +648,34 @@ struct rts_batch_plan {
 
 namespace rts {
 
-// Tile directory of one column for tiles of `tile_rows` rows (the index's own directory is
-// for kTileRows): first B-list entry of bin k with row >= t * tile_rows.
-__global__ void k_retile_directory(const uint2 *bent, const uint32_t *boff, int64_t ntiles,
-                                   int64_t tile_rows, uint32_t *bdir) {
-  const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t per = ntiles + 1;
-  if (id >= per * kBins) return;
-  const int k = (int)(id / per);
-  const uint64_t row = (uint64_t)(id % per) * (uint64_t)tile_rows;
-  uint32_t lo = boff[k], hi = boff[k + 1];
-  while (lo < hi) {
-    const uint32_t mid = lo + (hi - lo) / 2;
-    if (bent[mid].x < row)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  bdir[id] = lo;
-}
-
 static bool valid_config(const rts_batch_config &c) {
   return (c.tile_words == 512 || c.tile_words == 1024 || c.tile_words == 2048) && (c.order == 0 || c.order == 1) &&
          (c.threads == 256 || c.threads == 512) &&
          c.queries_per_group >= 1 &&
-         c.queries_per_group <= kBatchMaxGroup && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16;
+         c.queries_per_group <= kBatchMaxGroup && c.blocks_per_cu >= 1 && c.blocks_per_cu <= 16 &&
+         c.bins >= 0 && c.direct_weight >= 0 && c.direct_weight <= 1000000;
+}
+
+// Index set a config selects: bins == 0 the first, else the set with that bin count.
+static int set_for(const rts_batch_plan *p, const rts_batch_config &c) {
+  if (c.bins == 0) return 0;
+  for (size_t i = 0; i < p->set_bins.size(); i++)
+    if (p->set_bins[i] == c.bins) return (int)i;
+  return -1;
+}
+
+// Engine bins of a set of indexes (0 when no column has one, -1 when they differ or an
+// index does not cover n rows).
+static int bins_of_set(const BinDex *const *indexes, int ncols, int64_t n) {
+  int bins = 0;
+  for (int c = 0; indexes && c < ncols; c++) {
+    const BinDex *b = indexes[c];
+    if (!b) continue;
+    if (!b->dev || b->dev->n != n) return -1;
+    if (bins && b->dev->bins != bins) return -1;
+    bins = b->dev->bins;
+  }
+  return bins;
 }
 
 static void free_dirs(rts_batch_plan *p) {
@@ -669,12 +699,10 @@ static hipError_t build_dirs(rts_batch_plan *p, hipStream_t s) {
     const BinDex *b = p->index[c];
     if (!b || !b->dev) continue;
     const rts_bindex_dev *d = b->dev;
-    const int64_t items = (int64_t)kBins * (ntiles + 1);
+    const int64_t items = (int64_t)d->bins * (ntiles + 1);
     hipError_t e = hipMalloc((void **)&p->dir_base[c], items * sizeof(uint32_t));
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(k_retile_directory, dim3((unsigned)((items + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, d->bent, d->boff, ntiles, tile_rows, p->dir_base[c]);
-    e = hipGetLastError();
+    e = launch_tile_directory(d, ntiles, tile_rows, p->dir_base[c], s);
     if (e != hipSuccess) return e;
   }
   p->ntiles = ntiles;
@@ -739,6 +767,8 @@ void rts_batch_default_config(rts_batch_config *cfg) {
   cfg->blocks_per_cu = 4;
   cfg->order = 1;
   cfg->threads = 256;
+  cfg->bins = 0;
+  cfg->direct_weight = 0;
 }
 
 hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *columns,
@@ -752,21 +782,22 @@ hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *column
   p->words = (n + 31) / 32;
   p->max_queries = max_queries;
   p->columns.assign(columns, columns + ncols);
-  p->index_store.assign(ncols, nullptr);
-  for (int c = 0; c < ncols; c++) {
-    const BinDex *b = indexes ? indexes[c] : nullptr;
-    if (b && (!b->dev || b->dev->n != n)) {
-      delete p;
-      return set_error(hipErrorInvalidValue, "rts_batch_plan_create: index length");
-    }
-    p->index_store[c] = b;
+  const int bins = bins_of_set(indexes, ncols, n);
+  if (bins < 0) {
+    delete p;
+    return set_error(hipErrorInvalidValue, "rts_batch_plan_create: indexes (length or bin counts differ)");
   }
-  p->index = p->index_store.data();
+  p->index_sets.emplace_back(ncols, nullptr);
+  for (int c = 0; indexes && c < ncols; c++) p->index_sets[0][c] = indexes[c];
+  p->set_bins.push_back(bins);
+  p->index = p->index_sets[0].data();
   if (cfg) p->cfg = *cfg; else rts_batch_default_config(&p->cfg);
-  if (!valid_config(p->cfg)) {
+  if (!valid_config(p->cfg) || set_for(p, p->cfg) < 0) {
     delete p;
     return set_error(hipErrorInvalidValue, "rts_batch_plan_create: config");
   }
+  p->active_set = set_for(p, p->cfg);
+  p->index = p->index_sets[p->active_set].data();
   hipError_t e = hipMalloc((void **)&p->d_groups, (size_t)max_queries * sizeof(BatchGroup));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_ngroups, sizeof(int));
   if (e == hipSuccess) e = hipMalloc((void **)&p->d_sched, kSchedWords * sizeof(unsigned long long));
@@ -785,10 +816,19 @@ hipError_t rts_batch_plan_create(rts_batch_plan **out, const CODE *const *column
 
 hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *cfg,
                                      hipStream_t stream) {
-  if (!p || !cfg || !valid_config(*cfg)) return set_error(hipErrorInvalidValue, "rts_batch_plan_set_config");
-  const bool retile = cfg->tile_words != p->cfg.tile_words;
-  const bool reorder = cfg->order != p->cfg.order || cfg->threads != p->cfg.threads;
+  if (!p || !cfg || !valid_config(*cfg) || set_for(p, *cfg) < 0)
+    return set_error(hipErrorInvalidValue, "rts_batch_plan_set_config");
+  const int set = set_for(p, *cfg);
+  // a new tiling or index set needs new list directories
+  const bool retile = cfg->tile_words != p->cfg.tile_words || set != p->active_set;
+  // the grouping depends on the order, the slots a workgroup's LDS holds, the group size
+  // and which query-columns the cost model sends to the direct form
+  const bool reorder = cfg->order != p->cfg.order || cfg->threads != p->cfg.threads ||
+                       cfg->queries_per_group != p->cfg.queries_per_group ||
+                       cfg->direct_weight != p->cfg.direct_weight;
   p->cfg = *cfg;
+  p->active_set = set;
+  p->index = p->index_sets[set].data();
   if (p->exec) {
     (void)hipGraphExecDestroy(p->exec);
     p->exec = nullptr;
@@ -813,6 +853,25 @@ hipError_t rts_batch_plan_set_config(rts_batch_plan *p, const rts_batch_config *
                                         (int)specs.size(), stream);
 }
 
+hipError_t rts_batch_plan_add_indexes(rts_batch_plan *p, const BinDex *const *indexes, int *bins) {
+  if (!p || !indexes) return set_error(hipErrorInvalidValue, "rts_batch_plan_add_indexes");
+  const int b = bins_of_set(indexes, p->ncols, p->n);
+  if (b <= 0) return set_error(hipErrorInvalidValue, "rts_batch_plan_add_indexes: indexes");
+  for (size_t i = 0; i < p->index_sets.size(); i++) {
+    if (p->set_bins[i] != b) continue;
+    // replace the set with that bin count (not the active one: its directories are in use)
+    if ((int)i == p->active_set) return set_error(hipErrorInvalidValue, "rts_batch_plan_add_indexes: active set");
+    p->index_sets[i].assign(indexes, indexes + p->ncols);
+    if (bins) *bins = b;
+    return hipSuccess;
+  }
+  p->index_sets.emplace_back(indexes, indexes + p->ncols);
+  p->set_bins.push_back(b);
+  p->index = p->index_sets[p->active_set].data();  // the vector of sets may have moved
+  if (bins) *bins = b;
+  return hipSuccess;
+}
+
 hipError_t rts_batch_plan_set_queries(rts_batch_plan *p, const rts_query_spec *queries, int nq,
                                       hipStream_t stream) {
   return rts_batch_plan_set_queries_out(p, queries, nullptr, nq, stream);
@@ -826,6 +885,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
     if (!outs[q] || ((uintptr_t)outs[q] & 15))
       return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: outputs must be 16-byte aligned");
   // 1. every query-column in BinDex or direct form
+  p->stats_auto_direct = 0;
   std::vector<PlannedCol> pc((size_t)nq * kBatchMaxCols);
   std::vector<int> qcols(nq, 0), qempty(nq, 0);
   int st_bindex = 0, st_direct = 0, st_narrow = 0;
@@ -844,8 +904,19 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       const BinDex *b = p->index[col];
       if (in.mode[c] == RTS_EVAL_BINDEX && !b)
         return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: column has no index");
-      if (in.mode[c] == RTS_EVAL_DIRECT || !b) {
-        if (((uintptr_t)p->columns[col] & 15) != 0)
+      const bool aligned = ((uintptr_t)p->columns[col] & 15) == 0;
+      BoundSpec sl{-1, -1, 0}, sh{-1, -1, 0};
+      bool direct = in.mode[c] == RTS_EVAL_DIRECT || !b;
+      if (!direct) {
+        sl = lo == 0 ? BoundSpec{-1, -1, 0} : bound_spec(b->dev, lo, false);
+        sh = bound_spec(b->dev, hi + 1u, hi == 0xFFFFFFFFu);
+        if (in.mode[c] == RTS_EVAL_AUTO && aligned && direct_cheaper(b->dev, sl, sh, p->cfg.direct_weight)) {
+          direct = true;
+          p->stats_auto_direct++;
+        }
+      }
+      if (direct) {
+        if (!aligned)
           return set_error(hipErrorInvalidValue, "rts_batch_plan_set_queries: direct column not 16-byte aligned");
         P.direct = true;
         P.dlo = lo;
@@ -854,8 +925,6 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
         continue;
       }
       const rts_bindex_dev *d = b->dev;
-      const BoundSpec sl = lo == 0 ? BoundSpec{-1, -1, 0} : bound_spec(d, lo, false);
-      const BoundSpec sh = bound_spec(d, hi + 1u, hi == 0xFFFFFFFFu);
       P.lo = plan_bound(d, sl);
       P.hi = plan_bound(d, sh);
       // Both bounds in one bin: the result is the bin's list rows in [lo, hi] alone,
@@ -882,7 +951,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       std::vector<std::array<int, 3>> cols;
       for (int c = 0; c < qcols[q]; c++) {
         const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
-        cols.push_back(P.direct ? std::array<int, 3>{P.col, kBins, kBins}
+        cols.push_back(P.direct ? std::array<int, 3>{P.col, kMaxBins, kMaxBins}
                                 : std::array<int, 3>{P.col, P.lo.fvk, P.hi.fvk});
       }
       std::sort(cols.begin(), cols.end());
@@ -1015,6 +1084,58 @@ void rts_batch_plan_stats(const rts_batch_plan *p, int *bindex_cols, int *direct
   if (direct_cols) *direct_cols = p->stats_direct_cols;
   if (narrow_cols) *narrow_cols = p->stats_narrow_cols;
   if (ntiles) *ntiles = p->ntiles;
+}
+
+int rts_batch_plan_auto_direct(const rts_batch_plan *p) { return p ? p->stats_auto_direct : -1; }
+
+unsigned long long rts_batch_plan_mix_digest(const rts_batch_plan *p) {
+  if (!p) return 0;
+  std::map<std::vector<int>, int> cells;
+  for (const rts_query_spec &q : p->specs) {
+    std::vector<std::array<int, 2>> cols;
+    for (int c = 0; c < q.ncols && c < kBatchMaxCols; c++) {
+      const int col = q.column[c];
+      const CODE lo = q.lo[c], hi = q.hi[c];
+      double frac = 0;
+      const BinDex *b = col >= 0 && col < p->ncols ? p->index[col] : nullptr;
+      if (lo > hi) {
+        frac = 0;
+      } else if (b && b->dev) {
+        // areas (the reference's 128, the same for every bin count) the range touches
+        const rts_bindex_dev *d = b->dev;
+        const int a0 = lo <= d->starts[0] ? 0 : bin_of(d->starts, kAreas, lo);
+        const int a1 = hi == 0xFFFFFFFFu || hi + 1u > d->starts[kAreas - 1] ? kAreas - 1 : bin_of(d->starts, kAreas, hi);
+        frac = (double)(a1 - a0 + 1) / kAreas;
+      } else {
+        frac = ((double)hi - (double)lo + 1.0) / 4294967296.0;
+      }
+      const int bucket = frac <= 0 ? -64 : std::max(-32, (int)std::floor(std::log2(frac)));
+      cols.push_back({q.mode[c], bucket});
+    }
+    std::sort(cols.begin(), cols.end());
+    std::vector<int> key{q.ncols};
+    for (const auto &a : cols) key.insert(key.end(), a.begin(), a.end());
+    cells[key]++;
+  }
+  uint64_t h = 0xCBF29CE484222325ull;
+  auto mixin = [&h](int v) {
+    for (int i = 0; i < 4; i++) h = (h ^ (uint8_t)(v >> (8 * i))) * 0x100000001B3ull;
+  };
+  const int nq = std::max<int>(1, (int)p->specs.size());
+  for (const auto &cell : cells) {
+    const int share = (int)std::lround(64.0 * cell.second / nq);
+    if (!share) continue;
+    for (int v : cell.first) mixin(v);
+    mixin(-1000);
+    mixin(share);
+  }
+  return (unsigned long long)h;
+}
+
+int rts_batch_plan_groups(const rts_batch_plan *p, rts_batch_config *cfg) {
+  if (!p) return -1;
+  if (cfg) *cfg = p->cfg;
+  return p->ngroups;
 }
 
 void rts_batch_plan_info(const rts_batch_plan *p, long long *n, int *ncols, int *nq) {

@@ -4,8 +4,9 @@
 // The reference sorts on the host with an unstable parallel sort, builds the 127 filter
 // vectors with 127 host passes, then issues one cudaMalloc + cudaMemcpy per 3276-row block
 // (~30k per column at 1e8 rows).  Here everything stays on the device: one stable radix
-// sort of (value, row), a K-thread kernel for the area starts, one pass over the column
-// for all 127 filter vectors and the bin keys, one 8-bit stable radix sort for the
+// sort of (value, row), a K-thread kernel for the area starts and one for the bin starts
+// (each area cut into `sub` equal-count bins), one pass over the column for all bins-1
+// filter vectors and the bin keys, one stable radix sort of the bin keys for the
 // row-ordered B lists, and a directory kernel.
 #include <hipcub/hipcub.hpp>
 
@@ -29,8 +30,8 @@ __global__ void k_iota(uint32_t *p, int64_t n) {
 // start the area begins mid-run at i * avg, otherwise at the first occurrence of the value.
 __global__ void k_area_starts(const CODE *sorted, int64_t n, CODE *starts, int64_t *idx) {
   const int i = threadIdx.x;
-  if (i >= kBins) return;
-  const int64_t avg = n / kBins;
+  if (i >= kAreas) return;
+  const int64_t avg = n / kAreas;
   const CODE v = sorted[i * avg];
   starts[i] = v;
   if (i == 0) {
@@ -52,15 +53,26 @@ __global__ void k_area_starts(const CODE *sorted, int64_t n, CODE *starts, int64
   idx[i] = hi;
 }
 
-// One pass over the column: all K-1 filter vectors (fv[k] bit i = x_i < start[k+1],
+// Bin starts: area a (sorted slots [idx[a], idx[a + 1])) cut into `sub` equal-count bins;
+// bin a * sub + j starts at the value in slot idx[a] + j * len / sub, so bin a * sub is
+// the area's own start.
+__global__ void k_bin_starts(const CODE *sorted, int64_t n, const int64_t *idx, int sub, CODE *estarts) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= kAreas * sub) return;
+  const int a = b / sub, j = b % sub;
+  const int64_t s0 = idx[a], len = (a + 1 < kAreas ? idx[a + 1] : n) - s0;
+  estarts[b] = sorted[s0 + (int64_t)j * len / sub];
+}
+
+// One pass over the column: all bins-1 filter vectors (fv[k] bit i = x_i < estart[k+1],
 // set_fv_val_less, bindex.cpp:553-560) and each row's refine key (its bin if
-// start[bin] < x, else kBins = no refine).  One row per lane; a 64-row chunk gives two
+// estart[bin] < x, else `bins` = no refine).  One row per lane; a 64-row chunk gives two
 // MSB-first words per filter vector from one ballot.
 __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n,
-                                                      const CODE *starts_g, BITS *fvs,
-                                                      int64_t fv_stride, uint8_t *keys) {
-  __shared__ CODE starts[kBins];
-  for (int i = threadIdx.x; i < kBins; i += blockDim.x) starts[i] = starts_g[i];
+                                                      const CODE *starts_g, int bins, BITS *fvs,
+                                                      int64_t fv_stride, uint16_t *keys) {
+  __shared__ CODE starts[kMaxBins];
+  for (int i = threadIdx.x; i < bins; i += blockDim.x) starts[i] = starts_g[i];
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
@@ -70,8 +82,8 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
     const int64_t r = ch * 64 + lane;
     const bool valid = r < n;
     const CODE x = valid ? col[r] : 0;
-    // t = number of k in [1, K-1] with start[k] <= x (binary search in LDS)
-    int lo = 1, hi = kBins;
+    // t = number of k in [1, bins-1] with start[k] <= x (binary search in LDS)
+    int lo = 1, hi = bins;
     while (lo < hi) {
       const int mid = (lo + hi) >> 1;
       if (starts[mid] <= x)
@@ -80,10 +92,10 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
         hi = mid;
     }
     const int t = lo - 1;
-    if (valid) keys[r] = starts[t] < x ? (uint8_t)t : (uint8_t)kBins;
+    if (valid) keys[r] = starts[t] < x ? (uint16_t)t : (uint16_t)bins;
     const int64_t w0 = ch * 2;
     const bool w1_valid = (w0 + 1) * 32 < n;
-    for (int k = 0; k < kBins - 1; k++) {
+    for (int k = 0; k < bins - 1; k++) {
       const uint64_t b = __ballot(valid && k >= t);
       if (lane == 0) fvs[k * fv_stride + w0] = __brev((uint32_t)b);
       if (lane == 1 && w1_valid) fvs[k * fv_stride + w0 + 1] = __brev((uint32_t)(b >> 32));
@@ -100,10 +112,10 @@ __global__ void k_gather_entries(const CODE *col, const uint32_t *pos, uint2 *en
   }
 }
 
-// boff[k] = first sorted key >= k, for k in [0, kBins]
-__global__ void k_bin_offsets(const uint8_t *skeys, int64_t n, uint32_t *boff) {
-  const int k = threadIdx.x;
-  if (k > kBins) return;
+// boff[k] = first sorted key >= k, for k in [0, bins]
+__global__ void k_bin_offsets(const uint16_t *skeys, int64_t n, int bins, uint32_t *boff) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k > bins) return;
   int64_t lo = 0, hi = n;
   while (lo < hi) {
     const int64_t mid = lo + (hi - lo) / 2;
@@ -115,15 +127,14 @@ __global__ void k_bin_offsets(const uint8_t *skeys, int64_t n, uint32_t *boff) {
   boff[k] = (uint32_t)lo;
 }
 
-// bdir[k * (ntiles + 1) + t] = first entry of bin k with row >= t * kTileRows
-__global__ void k_tile_directory(const uint2 *bent, const uint32_t *boff, int64_t ntiles,
-                                 uint32_t *bdir) {
+// bdir[k * (ntiles + 1) + t] = first entry of bin k with row >= t * tile_rows
+__global__ void k_tile_directory(const uint2 *bent, const uint32_t *boff, int bins, int64_t ntiles,
+                                 int64_t tile_rows, uint32_t *bdir) {
   const int64_t id = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t per = ntiles + 1;
-  if (id >= per * kBins) return;
+  if (id >= per * bins) return;
   const int k = (int)(id / per);
-  const int64_t t = id % per;
-  const uint64_t row = (uint64_t)t * kTileRows;
+  const uint64_t row = (uint64_t)(id % per) * (uint64_t)tile_rows;
   uint32_t lo = boff[k], hi = boff[k + 1];
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
@@ -133,6 +144,14 @@ __global__ void k_tile_directory(const uint2 *bent, const uint32_t *boff, int64_
       hi = mid;
   }
   bdir[id] = lo;
+}
+
+hipError_t launch_tile_directory(const rts_bindex_dev *d, int64_t ntiles, int64_t tile_rows, uint32_t *bdir,
+                                 hipStream_t s) {
+  const int64_t items = (int64_t)d->bins * (ntiles + 1);
+  hipLaunchKernelGGL(k_tile_directory, dim3((unsigned)((items + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                     d->bent, d->boff, d->bins, ntiles, tile_rows, bdir);
+  return hipGetLastError();
 }
 
 // First and last value of every 3276-row block of every area (init_area,
@@ -179,7 +198,7 @@ static void free_dev(rts_bindex_dev *d) {
 static void make_block_views(const rts_bindex_dev *d, CODE *val, int32_t *pos, std::vector<pos_block> &blk,
                              std::vector<pos_block *> &ptrs) {
   blk.clear();
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     const int64_t len = d->area_len[k];
     const int nb = (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
     for (int b = 0; b < nb; b++) {
@@ -195,11 +214,11 @@ static void make_block_views(const rts_bindex_dev *d, CODE *val, int32_t *pos, s
 // The reference-shaped descriptor (bindex.h:126-138) over a complete device index: areas
 // and their blocks (bindex.cpp:633-652), start values, counts, filter-vector pointers.
 void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
-  d->area_store.resize(kBins);
-  d->area_gpu_store.resize(kBins);
+  d->area_store.resize(kAreas);
+  d->area_gpu_store.resize(kAreas);
   make_block_views(d, d->sorted_val, d->sorted_pos, d->blk_dev, d->blkp_dev);
   int64_t b0 = 0;
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     Area &a = d->area_store[k];
     const int64_t len = d->area_len[k];
     a.blockNum = (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
@@ -215,15 +234,15 @@ void describe(BinDex *bx, rts_bindex_dev *d, const CODE *col) {
   }
   bx->length = (POSTYPE)d->n;
   bx->data_min = d->starts[0];
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     bx->areas[k] = &d->area_store[k];
     bx->areaStartValues[k] = d->starts[k];
     bx->area_counts[k] = (POSTYPE)(d->area_start[k] + d->area_len[k]);
     bx->areasInGPU[k] = &d->area_gpu_store[k];
   }
-  for (int k = 0; k < kBins - 1; k++) {
+  for (int k = 0; k < kAreas - 1; k++) {
     bx->filterVectors[k] = nullptr;
-    bx->filterVectorsInGPU[k] = d->fvs + (size_t)k * d->fv_stride;
+    bx->filterVectorsInGPU[k] = d->fvs + (size_t)((k + 1) * d->sub - 1) * d->fv_stride;
   }
   bx->rawDataInGPU = const_cast<CODE *>(col);
   bx->dev = d;
@@ -246,29 +265,31 @@ hipError_t bindex_host_blocks(BinDex *bx) {
   }
   make_block_views(d, d->host_val.data(), d->host_pos.data(), d->blk_host, d->blkp_host);
   int64_t b0 = 0;
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     d->area_store[k].blocks = d->blkp_host.data() + b0;
     b0 += d->area_store[k].blockNum;
   }
   return hipSuccess;
 }
 
-hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s) {
+hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int sub, hipStream_t s) {
   hipError_t err = hipSuccess;
   rts_bindex_dev *d = nullptr;
   uint32_t *iota = nullptr, *bpos_in = nullptr;
-  uint8_t *keys = nullptr, *skeys = nullptr;
+  uint16_t *keys = nullptr, *skeys = nullptr;
   void *tmp = nullptr;
   size_t tmp_bytes = 0, need = 0;
-  CODE *starts_d = nullptr;
+  CODE *starts_d = nullptr, *estarts_d = nullptr;
   int64_t *idx_d = nullptr, *bf_slot_d = nullptr, *bl_slot_d = nullptr;
   CODE *bfirst_d = nullptr, *blast_d = nullptr;
   std::vector<int64_t> bf_slot, bl_slot;
   int64_t nblocks = 0;
-  int64_t idx_h[kBins];
+  int64_t idx_h[kAreas];
   uint32_t nb = 0;
+  int bins = 0, key_bits = 0;
 
-  if (!bx || !col || n < kBins || n > 0x7FFFFFFFll) return hipErrorInvalidValue;
+  if (!bx || !col || n < kAreas || n > 0x7FFFFFFFll || (sub != 1 && sub != 2 && sub != 4 && sub != 8))
+    return hipErrorInvalidValue;
   memset(bx, 0, sizeof(*bx));
   d = new rts_bindex_dev();
   d->n = n;
@@ -276,6 +297,9 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
   d->fv_stride = (d->words + kFvStrideAlign - 1) / kFvStrideAlign * kFvStrideAlign;
   d->ntiles = (n + kTileRows - 1) / kTileRows;
   d->owned_column = owned;
+  d->sub = sub;
+  d->bins = bins = kAreas * sub;
+  while ((1 << key_bits) <= bins) key_bits++;  // keys 0..bins
 
   // 1. stable sort of (value, row)
   RTS_TRY(hipMalloc((void **)&d->sorted_val, n * sizeof(CODE)));
@@ -290,42 +314,49 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
   RTS_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, need, col, d->sorted_val, iota,
                                              (uint32_t *)d->sorted_pos, (int)n, 0, 32, s));
 
-  // 2. area starts
-  RTS_TRY(hipMalloc((void **)&starts_d, kBins * sizeof(CODE)));
-  RTS_TRY(hipMalloc((void **)&idx_d, kBins * sizeof(int64_t)));
-  hipLaunchKernelGGL(k_area_starts, dim3(1), dim3(kBins), 0, s, d->sorted_val, n, starts_d, idx_d);
+  // 2. area starts, then bin starts
+  RTS_TRY(hipMalloc((void **)&starts_d, kAreas * sizeof(CODE)));
+  RTS_TRY(hipMalloc((void **)&idx_d, kAreas * sizeof(int64_t)));
+  RTS_TRY(hipMalloc((void **)&estarts_d, bins * sizeof(CODE)));
+  hipLaunchKernelGGL(k_area_starts, dim3(1), dim3(kAreas), 0, s, d->sorted_val, n, starts_d, idx_d);
   RTS_TRY(hipGetLastError());
+  hipLaunchKernelGGL(k_bin_starts, dim3((bins + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d->sorted_val, n,
+                     idx_d, sub, estarts_d);
+  RTS_TRY(hipGetLastError());
+  d->estarts.resize(bins);
   RTS_TRY(hipMemcpyAsync(d->starts, starts_d, sizeof(d->starts), hipMemcpyDeviceToHost, s));
+  RTS_TRY(hipMemcpyAsync(d->estarts.data(), estarts_d, bins * sizeof(CODE), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipMemcpyAsync(idx_h, idx_d, sizeof(idx_h), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipMemcpyAsync(&bx->data_max, d->sorted_val + n - 1, sizeof(CODE), hipMemcpyDeviceToHost, s));
 
   // 3. filter vectors + refine keys
-  RTS_TRY(hipMalloc((void **)&d->fvs, (size_t)(kBins - 1) * d->fv_stride * sizeof(BITS)));
-  RTS_TRY(hipMalloc((void **)&keys, n));
+  RTS_TRY(hipMalloc((void **)&d->fvs, (size_t)(bins - 1) * d->fv_stride * sizeof(BITS)));
+  RTS_TRY(hipMalloc((void **)&keys, n * sizeof(uint16_t)));
   hipLaunchKernelGGL(k_build_fvs, dim3(blocks_for((n + 63) / 64, kBlock / kWave)), dim3(kBlock), 0,
-                     s, col, n, starts_d, d->fvs, d->fv_stride, keys);
+                     s, col, n, estarts_d, bins, d->fvs, d->fv_stride, keys);
   RTS_TRY(hipGetLastError());
 
-  // 4. B lists: stable 8-bit sort of (key, row) -> rows grouped by bin in row order
-  RTS_TRY(hipMalloc((void **)&skeys, n));
+  // 4. B lists: stable sort of (key, row) -> rows grouped by bin in row order
+  RTS_TRY(hipMalloc((void **)&skeys, n * sizeof(uint16_t)));
   RTS_TRY(hipMalloc((void **)&bpos_in, n * sizeof(uint32_t)));
   RTS_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, need, keys, skeys, iota, bpos_in, (int)n, 0,
-                                             8, s));
+                                             key_bits, s));
   if (need > tmp_bytes) {
     (void)hipFree(tmp);
     tmp = nullptr;
     tmp_bytes = need;
     RTS_TRY(hipMalloc(&tmp, tmp_bytes));
   }
-  RTS_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, need, keys, skeys, iota, bpos_in, (int)n, 0, 8, s));
-  RTS_TRY(hipMalloc((void **)&d->boff, (kBins + 1) * sizeof(uint32_t)));
-  hipLaunchKernelGGL(k_bin_offsets, dim3(1), dim3(kBins + 1), 0, s, skeys, n, d->boff);
+  RTS_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, need, keys, skeys, iota, bpos_in, (int)n, 0, key_bits, s));
+  RTS_TRY(hipMalloc((void **)&d->boff, (bins + 1) * sizeof(uint32_t)));
+  hipLaunchKernelGGL(k_bin_offsets, dim3((bins + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, skeys, n, bins,
+                     d->boff);
   RTS_TRY(hipGetLastError());
-  d->boff_host.resize(kBins + 1);
-  RTS_TRY(hipMemcpyAsync(d->boff_host.data(), d->boff, (kBins + 1) * sizeof(uint32_t),
+  d->boff_host.resize(bins + 1);
+  RTS_TRY(hipMemcpyAsync(d->boff_host.data(), d->boff, (bins + 1) * sizeof(uint32_t),
                          hipMemcpyDeviceToHost, s));
   RTS_TRY(hipStreamSynchronize(s));
-  nb = d->boff_host[kBins];
+  nb = d->boff_host[bins];
   d->nb = nb;
   RTS_TRY(hipMalloc((void **)&d->bent, std::max<uint32_t>(nb, 1) * sizeof(uint2)));
   if (nb) {
@@ -333,20 +364,15 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hip
                        d->bent, (int64_t)nb);
     RTS_TRY(hipGetLastError());
   }
-  RTS_TRY(hipMalloc((void **)&d->bdir, (size_t)kBins * (d->ntiles + 1) * sizeof(uint32_t)));
-  {
-    const int64_t items = (int64_t)kBins * (d->ntiles + 1);
-    hipLaunchKernelGGL(k_tile_directory, dim3((unsigned)((items + kBlock - 1) / kBlock)), dim3(kBlock),
-                       0, s, d->bent, d->boff, d->ntiles, d->bdir);
-    RTS_TRY(hipGetLastError());
-  }
+  RTS_TRY(hipMalloc((void **)&d->bdir, (size_t)bins * (d->ntiles + 1) * sizeof(uint32_t)));
+  RTS_TRY(launch_tile_directory(d, d->ntiles, kTileRows, d->bdir, s));
 
   // 5. host descriptor: areas (bindex.cpp:633-652) and their blocks
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     d->area_start[k] = idx_h[k];
-    d->area_len[k] = (k < kBins - 1 ? idx_h[k + 1] : n) - idx_h[k];
+    d->area_len[k] = (k < kAreas - 1 ? idx_h[k + 1] : n) - idx_h[k];
   }
-  for (int k = 0; k < kBins; k++) {
+  for (int k = 0; k < kAreas; k++) {
     const int64_t len = d->area_len[k];
     int64_t i = 0;
     while (i + RTS_BLOCK_INIT_SIZE < len) {
@@ -383,6 +409,7 @@ fail:
   (void)hipFree(skeys);
   (void)hipFree(tmp);
   (void)hipFree(starts_d);
+  (void)hipFree(estarts_d);
   (void)hipFree(idx_d);
   (void)hipFree(bf_slot_d);
   (void)hipFree(bl_slot_d);
@@ -406,29 +433,73 @@ void bindex_free(BinDex *bx) {
 // ----------------------------------------------------------------- snapshot (persistence)
 //
 // File layout (host byte order, little-endian on MI355X hosts):
-//   "RTSBDX" 0 1 | u32 version, u32 bins | i64 n, words, fv_stride, ntiles, nb, nblocks |
-//   u32 data_max | starts[bins] u32 | area_start[bins] i64 | area_len[bins] i64 |
+//   "RTSBDX" 0 2 | u32 version, u32 bins, u32 sub, u32 pad | i64 n, words, fv_stride, ntiles,
+//   nb, nblocks | u32 data_max, u32 pad | u64 digests: host arrays, sorted_val, sorted_pos,
+//   fvs, bent |
+//   starts[128] u32 | area_start[128] i64 | area_len[128] i64 | estarts[bins] u32 |
 //   boff[bins + 1] u32 | block_first[nblocks] u32 | block_last[nblocks] u32 |
-//   sorted_val[n] u32 | sorted_pos[n] i32 | fvs[(bins - 1) * fv_stride] u32 | bent[nb] (u32, u32) |
-//   bdir[bins * (ntiles + 1)] u32
-// Device arrays stream through one page-locked staging buffer.
+//   sorted_val[n] u32 | sorted_pos[n] i32 | fvs[(bins - 1) * fv_stride] u32 | bent[nb] (u32, u32)
+// Device arrays stream through one page-locked staging buffer.  The tile directory is not
+// stored: the load rebuilds it from the B lists.  A load checks the header, the host arrays'
+// structure (areas tile the sorted column, block counts, starts non-decreasing, list offsets
+// non-decreasing up to nb), every B-list entry as it passes through the staging buffer (row
+// < n, rows increasing within a bin: the kernels index LDS by row), and the digest of every
+// array; any mismatch refuses the file before the index is used.
 
 namespace rts {
+
+hipError_t bitmap_digest(const BITS *bm, int64_t n, unsigned long long *digest, hipStream_t s);
+
 namespace {
 
-constexpr char kSnapMagic[8] = {'R', 'T', 'S', 'B', 'D', 'X', 0, 1};
-constexpr uint32_t kSnapVersion = 1;
+constexpr char kSnapMagic[8] = {'R', 'T', 'S', 'B', 'D', 'X', 0, 2};
+constexpr uint32_t kSnapVersion = 2;
 constexpr size_t kStage = 64u << 20;
 
 struct SnapHeader {
   char magic[8];
-  uint32_t version, bins;
+  uint32_t version, bins, sub, pad0;
   int64_t n, words, fv_stride, ntiles, nb, nblocks;
-  uint32_t data_max, pad;
+  uint32_t data_max, pad1;
+  uint64_t dg_host, dg_val, dg_pos, dg_fvs, dg_bent;
 };
 
 bool put(FILE *f, const void *p, size_t bytes) { return fwrite(p, 1, bytes, f) == bytes; }
 bool get(FILE *f, void *p, size_t bytes) { return fread(p, 1, bytes, f) == bytes; }
+
+// FNV-1a over host arrays, 8 bytes at a time (a few KB per index).
+struct HostDigest {
+  uint64_t h = 0xCBF29CE484222325ull;
+  void add(const void *p, size_t bytes) {
+    const unsigned char *c = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < bytes; i++) h = (h ^ c[i]) * 0x100000001B3ull;
+  }
+};
+
+uint64_t host_digest(const rts_bindex_dev *d) {
+  HostDigest g;
+  g.add(d->starts, sizeof(d->starts));
+  g.add(d->area_start, sizeof(d->area_start));
+  g.add(d->area_len, sizeof(d->area_len));
+  g.add(d->estarts.data(), d->estarts.size() * sizeof(CODE));
+  g.add(d->boff_host.data(), d->boff_host.size() * sizeof(uint32_t));
+  g.add(d->block_first.data(), d->block_first.size() * sizeof(CODE));
+  g.add(d->block_last.data(), d->block_last.size() * sizeof(CODE));
+  return g.h;
+}
+
+// Digest of a device array of `words` 32-bit words (the checker's bitmap digest over all
+// bits of every word).
+hipError_t device_digest(const void *p, int64_t words, uint64_t *out) {
+  if (words == 0) {
+    *out = 0;
+    return hipSuccess;
+  }
+  unsigned long long dg = 0;
+  hipError_t e = bitmap_digest(static_cast<const BITS *>(p), words * 32, &dg, ref_stream());
+  *out = dg;
+  return e;
+}
 
 hipError_t put_device(FILE *f, const void *dev, size_t bytes, void *stage) {
   for (size_t off = 0; off < bytes; off += kStage) {
@@ -440,14 +511,43 @@ hipError_t put_device(FILE *f, const void *dev, size_t bytes, void *stage) {
   return hipSuccess;
 }
 
-hipError_t get_device(FILE *f, void *dev, size_t bytes, void *stage) {
+// check(stage, offset, len) sees every chunk on the host before it is uploaded.
+template <class Check>
+hipError_t get_device(FILE *f, void *dev, size_t bytes, void *stage, Check check) {
   for (size_t off = 0; off < bytes; off += kStage) {
     const size_t len = std::min(kStage, bytes - off);
     if (!get(f, stage, len)) return hipErrorInvalidValue;
+    if (!check(stage, off, len)) return hipErrorInvalidValue;
     hipError_t e = hipMemcpy((char *)dev + off, stage, len, hipMemcpyHostToDevice);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;
+}
+
+const auto kNoCheck = [](const void *, size_t, size_t) { return true; };
+
+int blocks_of(int64_t len) {
+  return (int)(len <= RTS_BLOCK_INIT_SIZE ? 1 : (len + RTS_BLOCK_INIT_SIZE - 1) / RTS_BLOCK_INIT_SIZE);
+}
+
+// Structure of the host arrays read from a snapshot.
+bool host_arrays_valid(const rts_bindex_dev *d, int64_t nblocks) {
+  int64_t slot = 0, blocks = 0;
+  for (int k = 0; k < kAreas; k++) {
+    if (d->area_start[k] != slot || d->area_len[k] < 1) return false;
+    slot += d->area_len[k];
+    blocks += blocks_of(d->area_len[k]);
+    if (k && d->starts[k] < d->starts[k - 1]) return false;
+  }
+  if (slot != d->n || blocks != nblocks) return false;
+  for (int b = 0; b < d->bins; b++) {
+    if (b && d->estarts[b] < d->estarts[b - 1]) return false;
+    if (b % d->sub == 0 && d->estarts[b] != d->starts[b / d->sub]) return false;
+  }
+  if (d->boff_host[0] != 0 || d->boff_host[d->bins] != (uint32_t)d->nb) return false;
+  for (int b = 0; b < d->bins; b++)
+    if (d->boff_host[b + 1] < d->boff_host[b]) return false;
+  return true;
 }
 
 }  // namespace
@@ -455,14 +555,11 @@ hipError_t get_device(FILE *f, void *dev, size_t bytes, void *stage) {
 hipError_t bindex_save(const BinDex *bx, const char *path) {
   if (!bx || !bx->dev || !path) return hipErrorInvalidValue;
   const rts_bindex_dev *d = bx->dev;
-  FILE *f = fopen(path, "wb");
-  if (!f) return hipErrorInvalidValue;
-  void *stage = nullptr;
-  hipError_t e = hipHostMalloc(&stage, kStage, hipHostMallocDefault);
   SnapHeader h{};
   memcpy(h.magic, kSnapMagic, 8);
   h.version = kSnapVersion;
-  h.bins = kBins;
+  h.bins = (uint32_t)d->bins;
+  h.sub = (uint32_t)d->sub;
   h.n = d->n;
   h.words = d->words;
   h.fv_stride = d->fv_stride;
@@ -470,18 +567,27 @@ hipError_t bindex_save(const BinDex *bx, const char *path) {
   h.nb = d->nb;
   h.nblocks = (int64_t)d->block_first.size();
   h.data_max = bx->data_max;
-  const size_t dir = (size_t)kBins * (d->ntiles + 1);
+  h.dg_host = host_digest(d);
+  const size_t fv_words = (size_t)(d->bins - 1) * d->fv_stride;
+  hipError_t e = device_digest(d->sorted_val, d->n, &h.dg_val);
+  if (e == hipSuccess) e = device_digest(d->sorted_pos, d->n, &h.dg_pos);
+  if (e == hipSuccess) e = device_digest(d->fvs, (int64_t)fv_words, &h.dg_fvs);
+  if (e == hipSuccess) e = device_digest(d->bent, 2 * d->nb, &h.dg_bent);
+  if (e != hipSuccess) return e;
+  FILE *f = fopen(path, "wb");
+  if (!f) return hipErrorInvalidValue;
+  void *stage = nullptr;
+  e = hipHostMalloc(&stage, kStage, hipHostMallocDefault);
   bool ok = e == hipSuccess && put(f, &h, sizeof(h)) && put(f, d->starts, sizeof(d->starts)) &&
             put(f, d->area_start, sizeof(d->area_start)) && put(f, d->area_len, sizeof(d->area_len)) &&
-            put(f, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t)) &&
+            put(f, d->estarts.data(), d->bins * sizeof(CODE)) &&
+            put(f, d->boff_host.data(), (d->bins + 1) * sizeof(uint32_t)) &&
             put(f, d->block_first.data(), h.nblocks * sizeof(CODE)) &&
             put(f, d->block_last.data(), h.nblocks * sizeof(CODE));
   if (ok) e = put_device(f, d->sorted_val, d->n * sizeof(CODE), stage);
   if (ok && e == hipSuccess) e = put_device(f, d->sorted_pos, d->n * sizeof(int32_t), stage);
-  if (ok && e == hipSuccess)
-    e = put_device(f, d->fvs, (size_t)(kBins - 1) * d->fv_stride * sizeof(BITS), stage);
+  if (ok && e == hipSuccess) e = put_device(f, d->fvs, fv_words * sizeof(BITS), stage);
   if (ok && e == hipSuccess) e = put_device(f, d->bent, d->nb * sizeof(uint2), stage);
-  if (ok && e == hipSuccess) e = put_device(f, d->bdir, dir * sizeof(uint32_t), stage);
   if (stage) (void)hipHostFree(stage);
   if (fclose(f) != 0) ok = false;
   if (e != hipSuccess) return e;
@@ -496,11 +602,28 @@ hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
   rts_bindex_dev *d = nullptr;
   void *stage = nullptr;
   hipError_t e = hipErrorInvalidValue;
-  size_t dir = 0;
+  size_t fv_words = 0;
+  uint64_t dg = 0;
+  int64_t bin = 0;        // bin of the next B-list entry to check
+  uint32_t prev_row = 0;  // row of the previous entry of that bin
+  auto check_entries = [&](const void *p, size_t off, size_t len) {
+    if (off % sizeof(uint2) || len % sizeof(uint2)) return false;
+    const uint2 *ent = static_cast<const uint2 *>(p);
+    for (size_t i = 0, e0 = off / sizeof(uint2); i < len / sizeof(uint2); i++) {
+      const uint64_t e = e0 + i;
+      while (bin < d->bins && e >= d->boff_host[bin + 1]) bin++;
+      if (bin >= d->bins || ent[i].x >= (uint64_t)d->n) return false;
+      if (e > d->boff_host[bin] && ent[i].x <= prev_row) return false;  // row order in a bin
+      prev_row = ent[i].x;
+    }
+    return true;
+  };
   if (!get(f, &h, sizeof(h)) || memcmp(h.magic, kSnapMagic, 8) != 0 || h.version != kSnapVersion ||
-      h.bins != (uint32_t)kBins || h.n < kBins || h.n > 0x7FFFFFFFll || h.words != (h.n + 31) / 32 ||
-      h.fv_stride < h.words || h.ntiles != (h.n + kTileRows - 1) / kTileRows || h.nb < 0 || h.nb > h.n ||
-      h.nblocks < kBins || h.nblocks > h.n)
+      (h.sub != 1 && h.sub != 2 && h.sub != 4 && h.sub != 8) || h.bins != (uint32_t)kAreas * h.sub ||
+      h.n < kAreas || h.n > 0x7FFFFFFFll || h.words != (h.n + 31) / 32 || h.fv_stride < h.words ||
+      h.fv_stride % kFvStrideAlign != 0 || h.fv_stride > h.words + kFvStrideAlign ||
+      h.ntiles != (h.n + kTileRows - 1) / kTileRows || h.nb < 0 || h.nb > h.n || h.nblocks < kAreas ||
+      h.nblocks > h.n)
     goto done;
   d = new rts_bindex_dev();
   d->n = h.n;
@@ -508,30 +631,44 @@ hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
   d->fv_stride = h.fv_stride;
   d->ntiles = h.ntiles;
   d->nb = h.nb;
-  d->boff_host.resize(kBins + 1);
+  d->sub = (int)h.sub;
+  d->bins = (int)h.bins;
+  d->estarts.resize(d->bins);
+  d->boff_host.resize(d->bins + 1);
   d->block_first.resize(h.nblocks);
   d->block_last.resize(h.nblocks);
   if (!get(f, d->starts, sizeof(d->starts)) || !get(f, d->area_start, sizeof(d->area_start)) ||
-      !get(f, d->area_len, sizeof(d->area_len)) || !get(f, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t)) ||
+      !get(f, d->area_len, sizeof(d->area_len)) || !get(f, d->estarts.data(), d->bins * sizeof(CODE)) ||
+      !get(f, d->boff_host.data(), (d->bins + 1) * sizeof(uint32_t)) ||
       !get(f, d->block_first.data(), h.nblocks * sizeof(CODE)) ||
       !get(f, d->block_last.data(), h.nblocks * sizeof(CODE)))
     goto done;
-  if (d->boff_host[kBins] != (uint32_t)h.nb) goto done;
-  dir = (size_t)kBins * (h.ntiles + 1);
+  if (host_digest(d) != h.dg_host || !host_arrays_valid(d, h.nblocks)) goto done;
+  fv_words = (size_t)(d->bins - 1) * h.fv_stride;
   e = hipHostMalloc(&stage, kStage, hipHostMallocDefault);
   if (e == hipSuccess) e = hipMalloc((void **)&d->sorted_val, h.n * sizeof(CODE));
   if (e == hipSuccess) e = hipMalloc((void **)&d->sorted_pos, h.n * sizeof(int32_t));
-  if (e == hipSuccess) e = hipMalloc((void **)&d->fvs, (size_t)(kBins - 1) * h.fv_stride * sizeof(BITS));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->fvs, fv_words * sizeof(BITS));
   if (e == hipSuccess) e = hipMalloc((void **)&d->bent, std::max<int64_t>(h.nb, 1) * sizeof(uint2));
-  if (e == hipSuccess) e = hipMalloc((void **)&d->boff, (kBins + 1) * sizeof(uint32_t));
-  if (e == hipSuccess) e = hipMalloc((void **)&d->bdir, dir * sizeof(uint32_t));
-  if (e == hipSuccess) e = get_device(f, d->sorted_val, h.n * sizeof(CODE), stage);
-  if (e == hipSuccess) e = get_device(f, d->sorted_pos, h.n * sizeof(int32_t), stage);
-  if (e == hipSuccess) e = get_device(f, d->fvs, (size_t)(kBins - 1) * h.fv_stride * sizeof(BITS), stage);
-  if (e == hipSuccess) e = get_device(f, d->bent, h.nb * sizeof(uint2), stage);
-  if (e == hipSuccess) e = get_device(f, d->bdir, dir * sizeof(uint32_t), stage);
+  if (e == hipSuccess) e = hipMalloc((void **)&d->boff, (d->bins + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = hipMalloc((void **)&d->bdir, (size_t)d->bins * (h.ntiles + 1) * sizeof(uint32_t));
+  if (e == hipSuccess) e = get_device(f, d->sorted_val, h.n * sizeof(CODE), stage, kNoCheck);
+  if (e == hipSuccess) e = get_device(f, d->sorted_pos, h.n * sizeof(int32_t), stage, kNoCheck);
+  if (e == hipSuccess) e = get_device(f, d->fvs, fv_words * sizeof(BITS), stage, kNoCheck);
+  if (e == hipSuccess) e = get_device(f, d->bent, h.nb * sizeof(uint2), stage, check_entries);
+  // the arrays as uploaded must carry the digests the saved index had
+  if (e == hipSuccess) e = device_digest(d->sorted_val, h.n, &dg);
+  if (e == hipSuccess && dg != h.dg_val) e = hipErrorInvalidValue;
+  if (e == hipSuccess) e = device_digest(d->sorted_pos, h.n, &dg);
+  if (e == hipSuccess && dg != h.dg_pos) e = hipErrorInvalidValue;
+  if (e == hipSuccess) e = device_digest(d->fvs, (int64_t)fv_words, &dg);
+  if (e == hipSuccess && dg != h.dg_fvs) e = hipErrorInvalidValue;
+  if (e == hipSuccess) e = device_digest(d->bent, 2 * h.nb, &dg);
+  if (e == hipSuccess && dg != h.dg_bent) e = hipErrorInvalidValue;
   if (e == hipSuccess)
-    e = hipMemcpy(d->boff, d->boff_host.data(), (kBins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+    e = hipMemcpy(d->boff, d->boff_host.data(), (d->bins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_tile_directory(d, d->ntiles, kTileRows, d->bdir, ref_stream());
+  if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e == hipSuccess) {
     memset(bx, 0, sizeof(*bx));
     bx->data_max = h.data_max;

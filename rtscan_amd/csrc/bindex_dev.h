@@ -1,20 +1,25 @@
 // bindex_dev.h -- the engine's device-resident BinDex index (behind BinDex::dev).
 //
-// Per column, K = 128 bins from the reference's equal-frequency area starts
-// (init_bindex, bindex.cpp:599-671):
+// Per column, the reference's K = 128 equal-frequency areas (init_bindex,
+// bindex.cpp:599-671), each cut into `sub` equal-count bins (sub = 1, 2, 4 or 8), so the
+// engine works with bins = 128 * sub bins whose boundaries include every area start:
 //   sorted_val / sorted_pos   the column sorted by value (stable: ties in row order), n each
-//   fvs                       K-1 filter vectors, fv[k] bit i = (x_i < start[k+1]),
-//                             fv_stride words apart (stride padded to 64 words)
-//   B lists                   for every bin k (value range [start[k], start[k+1])), the rows
-//                             with start[k] < x < start[k+1] in row order, one 8-byte entry
-//                             per row: bent[e] = (row, value); bin k's entries are
-//                             [boff[k], boff[k+1])
+//   estarts                   bin start values (host), estarts[a * sub] = area a's start
+//   fvs                       bins-1 filter vectors, fv[k] bit i = (x_i < estarts[k+1]),
+//                             fv_stride words apart (stride padded to 64 words); the
+//                             reference's filter vector of area a is fv[(a + 1) * sub - 1]
+//   B lists                   for every bin k (value range [estarts[k], estarts[k+1])), the
+//                             rows with estarts[k] < x < estarts[k+1] in row order, one
+//                             8-byte entry per row: bent[e] = (row, value); bin k's entries
+//                             are [boff[k], boff[k+1])
 //   bdir                      tile directory: bin k's first entry with row >= t * kTileRows
 //                             is bdir[k * (ntiles + 1) + t]
 // A bound v of a query (the set L(v) = {x < v}) is then one filter vector plus the rows of
 // one bin's B list with x >= v: L(v) = fv[t(v)] minus {B_t(v) rows with x >= v}, or
 // fv[t(v) - 1] exactly when v is a bin start.  Rows equal to a bin start never need a
-// refine, so a heavily duplicated value (Zipf) costs nothing at query time.
+// refine, so a heavily duplicated value (Zipf) costs nothing at query time.  More bins
+// mean shorter refine lists per bound (2n/bins entries at most) for (bins - 1) * n / 8
+// bytes of filter vectors.
 #pragma once
 
 #include <stdint.h>
@@ -26,7 +31,9 @@
 
 namespace rts {
 
-constexpr int kBins = RTS_VAREA_N;
+constexpr int kAreas = RTS_VAREA_N;            // the reference's K areas
+constexpr int kMaxSub = 8;                     // bins per area, at most
+constexpr int kMaxBins = kAreas * kMaxSub;     // 1024
 constexpr int kTileWords = 1024;              // bitmap words per query-kernel tile
 constexpr int kTileRows = kTileWords * 32;    // 32768 rows
 constexpr int kFvStrideAlign = 64;            // words
@@ -39,18 +46,21 @@ struct rts_bindex_dev {
   int64_t fv_stride = 0;
   int64_t ntiles = 0;
   int64_t nb = 0;  // total B-list entries
+  int sub = 1;     // bins per area
+  int bins = rts::kAreas;
   CODE *sorted_val = nullptr;
   int32_t *sorted_pos = nullptr;
   BITS *fvs = nullptr;
   uint2 *bent = nullptr;  // (row, value) per B-list entry
-  uint32_t *boff = nullptr;  // device, kBins + 1
-  uint32_t *bdir = nullptr;  // device, kBins * (ntiles + 1)
+  uint32_t *boff = nullptr;  // device, bins + 1
+  uint32_t *bdir = nullptr;  // device, bins * (ntiles + 1)
   CODE *owned_column = nullptr;
   // host copies
-  CODE starts[rts::kBins];
-  int64_t area_start[rts::kBins];
-  int64_t area_len[rts::kBins];
-  std::vector<uint32_t> boff_host;
+  CODE starts[rts::kAreas];  // the reference's area starts
+  int64_t area_start[rts::kAreas];
+  int64_t area_len[rts::kAreas];
+  std::vector<CODE> estarts;         // bins
+  std::vector<uint32_t> boff_host;   // bins + 1
   std::vector<Area> area_store;      // BinDex::areas
   std::vector<Area> area_gpu_store;  // BinDex::areasInGPU
   std::vector<CODE> block_first, block_last;
@@ -64,12 +74,12 @@ struct rts_bindex_dev {
 
 namespace rts {
 
-// Bin of value v: the number of k in [1, K-1] with start[k] <= v.
-inline int bin_of(const CODE *starts, CODE v) {
-  int lo = 1, hi = kBins;  // first k in [1, K) with start[k] > v
+// Bin of value v: the number of k in [1, bins-1] with estarts[k] <= v.
+inline int bin_of(const CODE *estarts, int bins, CODE v) {
+  int lo = 1, hi = bins;  // first k in [1, bins) with estarts[k] > v
   while (lo < hi) {
     const int mid = (lo + hi) / 2;
-    if (starts[mid] <= v)
+    if (estarts[mid] <= v)
       lo = mid + 1;
     else
       hi = mid;
@@ -79,17 +89,27 @@ inline int bin_of(const CODE *starts, CODE v) {
 
 // One query bound L(v) = {x < v} as (base filter vector, refine list).
 struct BoundSpec {
-  int fv;        // -1: no rows, kBins - 1: all rows, else filter vector index
+  int fv;        // -1: no rows, bins - 1: all rows, else filter vector index
   int list;      // -1: no refine, else bin whose B-list rows with x >= thr leave the set
   CODE thr;
 };
 
 inline BoundSpec bound_spec(const rts_bindex_dev *d, CODE v, bool infinite) {
-  if (infinite) return BoundSpec{kBins - 1, -1, 0};  // v = 2^32: every row
-  if (v <= d->starts[0]) return BoundSpec{-1, -1, 0};  // v <= min: no row is < v
-  const int k = bin_of(d->starts, v);
-  if (v == d->starts[k]) return BoundSpec{k - 1, -1, 0};
+  if (infinite) return BoundSpec{d->bins - 1, -1, 0};  // v = 2^32: every row
+  if (v <= d->estarts[0]) return BoundSpec{-1, -1, 0};  // v <= min: no row is < v
+  const int k = bin_of(d->estarts.data(), d->bins, v);
+  if (v == d->estarts[k]) return BoundSpec{k - 1, -1, 0};
   return BoundSpec{k, d->boff_host[k + 1] > d->boff_host[k] ? k : -1, v};
+}
+
+// Tile directory of an index for tiles of `tile_rows` rows: bdir[k * (ntiles + 1) + t] =
+// first B-list entry of bin k with row >= t * tile_rows (asynchronous on s).
+hipError_t launch_tile_directory(const rts_bindex_dev *d, int64_t ntiles, int64_t tile_rows, uint32_t *bdir,
+                                 hipStream_t s);
+
+// Refine-list entries of bin k.
+inline int64_t list_entries(const rts_bindex_dev *d, int k) {
+  return k < 0 ? 0 : (int64_t)d->boff_host[k + 1] - d->boff_host[k];
 }
 
 }  // namespace rts

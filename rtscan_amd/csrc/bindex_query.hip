@@ -17,7 +17,7 @@
 
 namespace rts {
 
-hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, hipStream_t s);
+hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int sub, hipStream_t s);
 void bindex_free(BinDex *bx);
 hipError_t bindex_save(const BinDex *bx, const char *path);
 hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col);
@@ -149,7 +149,7 @@ static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   BoundDev b{};
   if (s.fv < 0) {
     b.base = kBaseZero;
-  } else if (s.fv >= kBins - 1) {
+  } else if (s.fv >= d->bins - 1) {
     b.base = kBaseOnes;
   } else {
     b.base = kBaseFv;
@@ -311,12 +311,12 @@ extern "C" {
 // in_which_area, bindex.cpp:1138-1151 (dup bindex_cuda.cpp:723-731)
 int in_which_area(BinDex *bindex, CODE compare) {
   if (compare < bindex->areaStartValues[0]) return -1;
-  for (int i = 0; i < kBins; i++) {
+  for (int i = 0; i < kAreas; i++) {
     const CODE sv = bindex->areaStartValues[i];
     if (sv == compare) return i;
     if (sv > compare) return i - 1;
   }
-  return kBins - 1;
+  return kAreas - 1;
 }
 
 // in_which_block, bindex.cpp:1153-1176: the first block starting at `compare`, else the
@@ -359,7 +359,7 @@ void copy_filter_vector_in_GPU(BinDex *bindex, BITS *dev_bitmap, int k, bool neg
   hipError_t e;
   if (k < 0)
     e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0u, ref_stream());
-  else if (k >= kBins - 1)
+  else if (k >= kAreas - 1)
     e = bitop(5, dev_bitmap, nullptr, nullptr, words, 0xFFFFFFFFu, ref_stream());
   else
     e = bitop(negation ? 1 : 0, dev_bitmap, bindex->filterVectorsInGPU[k], nullptr, words, 0u, ref_stream());
@@ -432,10 +432,18 @@ hipError_t GPURefineEqAreaWithCuda(BinDex **bindexs, BITS *dev_result_bitmap, CO
 // ------------------------------------------------------------ additive BinDex API
 
 hipError_t rts_bindex_build(BinDex *bindex, const CODE *dev_column, POSTYPE n, hipStream_t stream) {
-  hipError_t e = bindex_build(bindex, dev_column, n, nullptr, stream);
-  if (e == hipSuccess) e = hipStreamSynchronize(stream);
-  return set_error(e, "rts_bindex_build");
+  return rts_bindex_build_bins(bindex, dev_column, n, RTS_VAREA_N, stream);
 }
+
+hipError_t rts_bindex_build_bins(BinDex *bindex, const CODE *dev_column, POSTYPE n, int bins,
+                                 hipStream_t stream) {
+  if (bins % RTS_VAREA_N != 0) return set_error(hipErrorInvalidValue, "rts_bindex_build_bins: bins");
+  hipError_t e = bindex_build(bindex, dev_column, n, nullptr, bins / RTS_VAREA_N, stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(stream);
+  return set_error(e, "rts_bindex_build_bins");
+}
+
+int rts_bindex_bins(const BinDex *bindex) { return bindex && bindex->dev ? bindex->dev->bins : -1; }
 
 hipError_t rts_bindex_save(const BinDex *bindex, const char *path) {
   return set_error(bindex_save(bindex, path), "rts_bindex_save");
@@ -450,7 +458,7 @@ hipError_t rts_bindex_build_host(BinDex *bindex, const CODE *host_column, POSTYP
   CODE *d = nullptr;
   hipError_t e = hipMalloc((void **)&d, (size_t)n * sizeof(CODE));
   if (e == hipSuccess) e = hipMemcpy(d, host_column, (size_t)n * sizeof(CODE), hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = bindex_build(bindex, d, n, d, ref_stream());
+  if (e == hipSuccess) e = bindex_build(bindex, d, n, d, 1, ref_stream());
   if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e != hipSuccess && d) (void)hipFree(d);
   return set_error(e, "rts_bindex_build_host");
@@ -486,8 +494,8 @@ long long rts_bindex_refine_entries(const BinDex *bindex) {
 long long rts_bindex_device_bytes(const BinDex *bindex) {
   if (!bindex || !bindex->dev) return -1;
   const rts_bindex_dev *d = bindex->dev;
-  return (long long)(d->n * 8 + (kBins - 1) * d->fv_stride * 4 + d->nb * 8 +
-                     (kBins + 1) * 4 + kBins * (d->ntiles + 1) * 4);
+  return (long long)(d->n * 8 + (int64_t)(d->bins - 1) * d->fv_stride * 4 + d->nb * 8 +
+                     (int64_t)(d->bins + 1) * 4 + (int64_t)d->bins * (d->ntiles + 1) * 4);
 }
 
 }  // extern "C"

@@ -54,23 +54,82 @@ int device_cu_count() {
   return cache[dev];
 }
 
-uint32_t *stream_count_partials(hipStream_t stream) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, uintptr_t, std::thread::id>, uint32_t *> table;
+// Scratch buffers of the calling host thread, keyed by (kind, device, stream).  Two threads
+// enqueueing work on the same stream (the legacy null stream, shared by all threads) must
+// not share scratch, since their launches can interleave between a kernel and the one that
+// reads its partials; so the table is thread-local.  A thread keeps at most
+// kScratchPerThread buffers (the oldest is freed first) and frees its buffers when it
+// exits, so thread pools and stream churn do not grow device memory without bound.
+// hipFree waits for the device's outstanding work, so a buffer is never freed under a
+// kernel still using it.  The main thread's buffers are left to process teardown.
+namespace {
+
+constexpr size_t kScratchPerThread = 16;
+
+struct ScratchEntry {
+  int kind, dev;
+  uintptr_t stream;
+  void *dptr, *hptr;
+};
+
+const std::thread::id g_main_thread = std::this_thread::get_id();
+
+struct ThreadScratch {
+  std::vector<ScratchEntry> entries;  // oldest first
+  static void release(ScratchEntry &e) {
+    int cur = 0;
+    const bool switched = hipGetDevice(&cur) == hipSuccess && cur != e.dev && hipSetDevice(e.dev) == hipSuccess;
+    if (e.dptr) (void)hipFree(e.dptr);
+    if (e.hptr) (void)hipHostFree(e.hptr);
+    if (switched) (void)hipSetDevice(cur);
+  }
+  ~ThreadScratch() {
+    if (std::this_thread::get_id() == g_main_thread) return;
+    for (ScratchEntry &e : entries) release(e);
+  }
+};
+
+thread_local ThreadScratch t_scratch;
+
+}  // namespace
+
+hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t host_bytes, void **dptr,
+                          void **hptr) {
   int dev = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return nullptr;
-  // One buffer per (device, stream, host thread): two threads enqueueing counted work on the
-  // same stream (the legacy null stream, shared by all threads, or hipStreamPerThread,
-  // which names a different stream on every thread) must not share partials, since their
-  // launches can interleave between a scan and its finalize.
-  const auto key = std::make_tuple(dev, (uintptr_t)stream, std::this_thread::get_id());
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = table.find(key);
-  if (it != table.end()) return it->second;
-  uint32_t *p = nullptr;
-  if (hipMalloc((void **)&p, kMaxCountBlocks * sizeof(uint32_t)) != hipSuccess) return nullptr;
-  table[key] = p;
-  return p;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::vector<ScratchEntry> &tab = t_scratch.entries;
+  for (const ScratchEntry &x : tab) {
+    if (x.kind == kind && x.dev == dev && x.stream == (uintptr_t)stream) {
+      *dptr = x.dptr;
+      if (hptr) *hptr = x.hptr;
+      return hipSuccess;
+    }
+  }
+  ScratchEntry x{kind, dev, (uintptr_t)stream, nullptr, nullptr};
+  if (dev_bytes) e = hipMalloc(&x.dptr, dev_bytes);
+  if (e == hipSuccess && host_bytes) e = hipHostMalloc(&x.hptr, host_bytes, 0);
+  if (e != hipSuccess) {
+    ThreadScratch::release(x);
+    return e;
+  }
+  if (tab.size() >= kScratchPerThread) {
+    ThreadScratch::release(tab.front());
+    tab.erase(tab.begin());
+  }
+  tab.push_back(x);
+  *dptr = x.dptr;
+  if (hptr) *hptr = x.hptr;
+  return hipSuccess;
+}
+
+size_t thread_scratch_entries() { return t_scratch.entries.size(); }
+
+uint32_t *stream_count_partials(hipStream_t stream) {
+  void *p = nullptr;
+  if (thread_scratch(kScratchCountPartials, stream, kMaxCountBlocks * sizeof(uint32_t), 0, &p, nullptr) != hipSuccess)
+    return nullptr;
+  return static_cast<uint32_t *>(p);
 }
 
 // One workgroup of 1024 threads; each thread keeps 8 independent loads in flight so the
@@ -345,5 +404,16 @@ hipError_t rts_device_info(char *name, size_t name_len, int *cu_count, size_t *t
 }
 
 const char *rts_last_error(void) { return g_last_error.c_str(); }
+
+int rts_thread_scratch_buffers(void) { return (int)thread_scratch_entries(); }
+
+hipError_t rts_device_mem_info(size_t *free_bytes, size_t *total_bytes) {
+  size_t f = 0, t = 0;
+  hipError_t e = hipMemGetInfo(&f, &t);
+  if (e != hipSuccess) return set_error(e, "rts_device_mem_info");
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return hipSuccess;
+}
 
 }  // extern "C"

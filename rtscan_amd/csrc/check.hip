@@ -188,8 +188,9 @@ void grid_for(uint64_t words, unsigned *blocks, uint64_t *chunk) {
 }
 
 // Per-block partials of one check, on the device and in pinned host memory, kept per
-// (device, stream, host thread) so that a check allocates nothing after its first call
-// (hipFree would also synchronise the device) and two threads never share them.
+// (device, stream, host thread) by thread_scratch, so that a check allocates nothing after
+// its first call (hipFree would also synchronise the device) and two threads never share
+// them.
 struct CheckScratch {
   void *dev = nullptr;
   void *host = nullptr;
@@ -199,28 +200,7 @@ struct CheckScratch {
 constexpr size_t kCheckScratchBytes = kCheckMaxBlocks * sizeof(ComparePart) + sizeof(rts_compare_result);
 
 hipError_t check_scratch(hipStream_t stream, CheckScratch *out) {
-  static std::mutex mu;
-  static std::map<std::tuple<int, uintptr_t, std::thread::id>, CheckScratch> table;
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  const auto key = std::make_tuple(dev, (uintptr_t)stream, std::this_thread::get_id());
-  std::lock_guard<std::mutex> lock(mu);
-  auto it = table.find(key);
-  if (it != table.end()) {
-    *out = it->second;
-    return hipSuccess;
-  }
-  CheckScratch c;
-  e = hipMalloc(&c.dev, kCheckScratchBytes);
-  if (e == hipSuccess) e = hipHostMalloc(&c.host, kCheckScratchBytes, 0);
-  if (e != hipSuccess) {
-    if (c.dev) (void)hipFree(c.dev);
-    return e;
-  }
-  table[key] = c;
-  *out = c;
-  return hipSuccess;
+  return thread_scratch(kScratchCheck, stream, kCheckScratchBytes, kCheckScratchBytes, &out->dev, &out->host);
 }
 
 }  // namespace

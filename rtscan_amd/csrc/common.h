@@ -103,8 +103,14 @@ __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink)
 // cudaDeviceSynchronize (cuda_and.cu:276-287).
 inline hipStream_t ref_stream() { return hipStreamPerThread; }
 
-// Scratch of (current device, stream, calling host thread) for count partials, allocated on
-// first use.
+// Scratch of (current device, stream, calling host thread), allocated on first use and
+// kept by the calling thread (a bounded number per thread, freed when the thread exits).
+// Not for use under graph capture: the first use on a stream allocates.
+enum ScratchKind : int { kScratchCountPartials = 0, kScratchCheck = 1 };
+hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t host_bytes, void **dptr,
+                          void **hptr);
+size_t thread_scratch_entries();
+// Count partials (kMaxCountBlocks slots) of (device, stream, calling thread).
 uint32_t *stream_count_partials(hipStream_t stream);
 
 // Sum partials[0, nblocks) into *out on `stream` (one small launch).

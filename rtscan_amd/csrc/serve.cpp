@@ -15,6 +15,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "admission.h"
 #include "rts_serve.h"
 
 namespace {
@@ -25,60 +26,23 @@ inline uint64_t now_ns() {
   return (uint64_t)std::chrono::duration_cast<std::chrono::nanoseconds>(Clock::now().time_since_epoch()).count();
 }
 
-// First-fit allocator over one device region, 256-byte granules, coalescing frees.
-class Pool {
- public:
-  bool init(size_t bytes) {
-    bytes_ = bytes / 256 * 256;
-    if (bytes_ == 0 || hipMalloc((void **)&base_, bytes_) != hipSuccess) return false;
-    free_[0] = bytes_;
-    return true;
-  }
+// The result pool: one device allocation; which ranges are free is the admission's
+// business (host only).
+struct DeviceRegion {
+  char *base = nullptr;
+  bool init(size_t bytes) { return hipMalloc((void **)&base, bytes) == hipSuccess; }
   void destroy() {
-    if (base_) (void)hipFree(base_);
-    base_ = nullptr;
+    if (base) (void)hipFree(base);
+    base = nullptr;
   }
-  // Offset of a free range of `bytes`, or SIZE_MAX.
-  size_t alloc(size_t bytes) {
-    bytes = (bytes + 255) / 256 * 256;
-    for (auto it = free_.begin(); it != free_.end(); ++it) {
-      if (it->second >= bytes) {
-        const size_t off = it->first, rest = it->second - bytes;
-        free_.erase(it);
-        if (rest) free_[off + bytes] = rest;
-        in_use_ += bytes;
-        peak_ = std::max(peak_, in_use_);
-        return off;
-      }
-    }
-    return SIZE_MAX;
-  }
-  void release(size_t off, size_t bytes) {
-    bytes = (bytes + 255) / 256 * 256;
-    in_use_ -= bytes;
-    auto it = free_.emplace(off, bytes).first;
-    auto next = std::next(it);
-    if (next != free_.end() && it->first + it->second == next->first) {
-      it->second += next->second;
-      free_.erase(next);
-    }
-    if (it != free_.begin()) {
-      auto prev = std::prev(it);
-      if (prev->first + prev->second == it->first) {
-        prev->second += it->second;
-        free_.erase(it);
-      }
-    }
-  }
-  BITS *at(size_t off) const { return reinterpret_cast<BITS *>(base_ + off); }
-  size_t capacity() const { return bytes_; }
-  size_t in_use() const { return in_use_; }
-  size_t peak() const { return peak_; }
+  BITS *at(size_t off) const { return reinterpret_cast<BITS *>(base + off); }
+};
 
- private:
-  char *base_ = nullptr;
-  size_t bytes_ = 0, in_use_ = 0, peak_ = 0;
-  std::map<size_t, size_t> free_;  // offset -> size
+// Completion tracking of one rts_server_replay call: its requests count here when they are
+// done or failed, so a replay returns only when none of them can still touch its arrays.
+struct ReplayState {
+  uint64_t finished = 0;
+  hipError_t error = hipSuccess;
 };
 
 struct Table {
@@ -99,6 +63,7 @@ struct Request {
   bool auto_release = false;
   uint64_t *lat_out = nullptr;
   unsigned long long *count_out = nullptr;
+  ReplayState *replay = nullptr;
 };
 
 struct Worker {
@@ -116,10 +81,10 @@ struct Worker {
 struct rts_server {
   rts_server_config cfg{};
   std::vector<Table> tables;
-  Pool pool;
+  DeviceRegion region;
+  rts::Admission adm;
   std::vector<Worker> workers;
   std::unordered_map<uint64_t, Request> reqs;
-  std::deque<uint64_t> queue;
   uint64_t next_ticket = 1;
   mutable std::mutex mu;
   std::condition_variable cv_loop, cv_done;
@@ -144,7 +109,8 @@ void complete_batch(rts_server *s, Worker &w, uint64_t t_done) {
     if (r.auto_release) {
       if (r.lat_out) *r.lat_out = t_done - r.arrival_ns;
       if (r.count_out) *r.count_out = r.count;
-      s->pool.release(r.off, r.bytes);
+      if (r.replay) r.replay->finished++;
+      s->adm.release(r.off, r.bytes);
       s->reqs.erase(it);
     }
     s->completed++;
@@ -153,49 +119,54 @@ void complete_batch(rts_server *s, Worker &w, uint64_t t_done) {
   w.busy = false;
 }
 
-// A batch whose work reported an error: its requests fail and their bitmaps return to the
-// pool (replayed ones count as completed, so a replay ends).
-void fail_batch(rts_server *s, Worker &w) {
-  for (uint64_t t : w.batch) {
+// Requests whose batch failed: their bitmaps return to the pool once the worker's stream
+// has drained (a kernel launched before the failure may still write them), replayed ones
+// are counted and dropped (so a replay ends), the others stay for rts_server_wait to report.
+void fail_requests(rts_server *s, Worker &w, const std::vector<uint64_t> &tickets, hipError_t e) {
+  (void)hipStreamSynchronize(w.stream);
+  s->loop_error = e;
+  for (uint64_t t : tickets) {
     auto it = s->reqs.find(t);
     if (it == s->reqs.end()) continue;
     Request &r = it->second;
     r.state = RTS_REQ_FAILED;
-    s->pool.release(r.off, r.bytes);
+    if (r.off != SIZE_MAX) s->adm.release(r.off, r.bytes);
     r.off = SIZE_MAX;
-    if (r.auto_release) s->reqs.erase(it);
+    if (r.auto_release) {
+      if (r.replay) {
+        r.replay->finished++;
+        r.replay->error = e;
+      }
+      s->reqs.erase(it);
+    }
     s->completed++;
   }
   w.batch.clear();
   w.busy = false;
+  s->cv_done.notify_all();
 }
 
 // Admit the oldest waiting request's table's requests (in arrival order) into one batch on
 // worker w.  Returns true when a batch was launched.
 bool launch_batch(rts_server *s, Worker &w) {
-  if (s->queue.empty()) return false;
-  const int table = s->reqs.at(s->queue.front()).table;
+  if (!s->adm.waiting()) return false;
+  const int table = s->adm.front_table();
+  const std::vector<rts::Admission::Admitted> got =
+      s->adm.admit(s->cfg.max_batch, [s](int t) { return s->tables[t].bitmap_bytes; });
+  if (got.empty()) return false;
   const Table &tb = s->tables[table];
   std::vector<uint64_t> admitted;
   std::vector<rts_query_spec> specs;
   std::vector<BITS *> outs;
-  for (auto it = s->queue.begin(); it != s->queue.end() && (int)admitted.size() < s->cfg.max_batch;) {
-    Request &r = s->reqs.at(*it);
-    if (r.table != table) {
-      ++it;
-      continue;
-    }
-    const size_t off = s->pool.alloc(tb.bitmap_bytes);
-    if (off == SIZE_MAX) break;  // memory: wait for releases, keep arrival order
-    r.off = off;
+  for (const auto &a : got) {
+    Request &r = s->reqs.at(a.ticket);
+    r.off = a.offset;
     r.bytes = tb.bitmap_bytes;
     r.state = RTS_REQ_RUNNING;
-    admitted.push_back(*it);
+    admitted.push_back(a.ticket);
     specs.push_back(r.spec);
-    outs.push_back(s->pool.at(off));
-    it = s->queue.erase(it);
+    outs.push_back(s->region.at(a.offset));
   }
-  if (admitted.empty()) return false;
   rts_batch_plan *plan = w.plans[table];
   hipError_t e = rts_batch_plan_set_queries_out(plan, specs.data(), outs.data(), (int)specs.size(), w.stream);
   if (e == hipSuccess) e = rts_batch_plan_launch(plan, nullptr, 0, w.dcounts, w.stream);
@@ -204,14 +175,7 @@ bool launch_batch(rts_server *s, Worker &w) {
                        hipMemcpyDeviceToHost, w.stream);
   if (e == hipSuccess) e = hipEventRecord(w.done, w.stream);
   if (e != hipSuccess) {
-    s->loop_error = e;
-    for (uint64_t t : admitted) {
-      Request &r = s->reqs.at(t);
-      r.state = RTS_REQ_FAILED;
-      s->pool.release(r.off, r.bytes);
-      r.off = SIZE_MAX;
-    }
-    s->cv_done.notify_all();
+    fail_requests(s, w, admitted, e);
     return false;
   }
   w.batch = admitted;
@@ -233,8 +197,8 @@ void loop_main(rts_server *s) {
         complete_batch(s, w, now_ns());
         progress = true;
       } else if (q != hipErrorNotReady) {  // the batch's work failed: fail its requests
-        s->loop_error = q;
-        fail_batch(s, w);
+        const std::vector<uint64_t> tickets = w.batch;
+        fail_requests(s, w, tickets, q);
         progress = true;
       }
     }
@@ -276,10 +240,12 @@ hipError_t rts_server_create(rts_server **out, const rts_server_config *cfg) {
     return hipErrorInvalidValue;
   }
   (void)hipGetDevice(&s->device);
-  if (!s->pool.init(s->cfg.pool_bytes)) {
+  const size_t pool = s->cfg.pool_bytes / rts::PoolAllocator::kGranule * rts::PoolAllocator::kGranule;
+  if (pool == 0 || !s->region.init(pool)) {
     delete s;
     return hipErrorOutOfMemory;
   }
+  s->adm.reset(pool);
   s->workers.resize(s->cfg.streams);
   for (Worker &w : s->workers) {
     hipError_t e = hipStreamCreateWithFlags(&w.stream, hipStreamNonBlocking);
@@ -330,25 +296,26 @@ hipError_t rts_server_start(rts_server *s) {
 }
 
 static hipError_t submit_at(rts_server *s, int table, const rts_query_spec *q, uint64_t arrival,
-                            uint64_t *lat_out, unsigned long long *count_out, bool auto_release,
+                            uint64_t *lat_out, unsigned long long *count_out, ReplayState *replay,
                             uint64_t *ticket) {
   if (!s || !q || table < 0 || table >= (int)s->tables.size() || q->ncols < 1 ||
       q->ncols > RTS_BATCH_MAX_COLS)
     return hipErrorInvalidValue;
   for (int c = 0; c < q->ncols; c++)
     if (q->column[c] < 0 || q->column[c] >= (int)s->tables[table].columns.size()) return hipErrorInvalidValue;
-  if (s->tables[table].bitmap_bytes > s->pool.capacity()) return hipErrorInvalidValue;
+  if (s->tables[table].bitmap_bytes > s->adm.pool().capacity()) return hipErrorInvalidValue;
   std::lock_guard<std::mutex> lock(s->mu);
-  if ((int)s->queue.size() >= s->cfg.queue_capacity) return hipErrorNotReady;
+  if ((int)s->adm.waiting() >= s->cfg.queue_capacity) return hipErrorNotReady;
   const uint64_t t = s->next_ticket++;
   Request &r = s->reqs[t];
   r.table = table;
   r.spec = *q;
   r.arrival_ns = arrival;
-  r.auto_release = auto_release;
+  r.auto_release = replay != nullptr;
   r.lat_out = lat_out;
   r.count_out = count_out;
-  s->queue.push_back(t);
+  r.replay = replay;
+  s->adm.enqueue(t, table);
   s->submitted++;
   if (ticket) *ticket = t;
   s->cv_loop.notify_one();
@@ -356,7 +323,7 @@ static hipError_t submit_at(rts_server *s, int table, const rts_query_spec *q, u
 }
 
 hipError_t rts_server_submit(rts_server *s, int table_id, const rts_query_spec *query, uint64_t *ticket) {
-  return submit_at(s, table_id, query, now_ns(), nullptr, nullptr, false, ticket);
+  return submit_at(s, table_id, query, now_ns(), nullptr, nullptr, nullptr, ticket);
 }
 
 int rts_server_wait(rts_server *s, uint64_t ticket, int timeout_ms, const BITS **bitmap,
@@ -375,7 +342,7 @@ int rts_server_wait(rts_server *s, uint64_t ticket, int timeout_ms, const BITS *
   if (it == s->reqs.end()) return RTS_REQ_FAILED;
   const Request &r = it->second;
   if (r.state == RTS_REQ_DONE) {
-    if (bitmap) *bitmap = s->pool.at(r.off);
+    if (bitmap) *bitmap = s->region.at(r.off);
     if (count) *count = r.count;
     if (latency_ns) *latency_ns = r.done_ns - r.arrival_ns;
   }
@@ -387,11 +354,8 @@ void rts_server_release(rts_server *s, uint64_t ticket) {
   std::lock_guard<std::mutex> lock(s->mu);
   auto it = s->reqs.find(ticket);
   if (it == s->reqs.end() || it->second.state == RTS_REQ_RUNNING) return;
-  if (it->second.off != SIZE_MAX) s->pool.release(it->second.off, it->second.bytes);
-  if (it->second.state == RTS_REQ_QUEUED) {
-    auto q = std::find(s->queue.begin(), s->queue.end(), ticket);
-    if (q != s->queue.end()) s->queue.erase(q);
-  }
+  if (it->second.off != SIZE_MAX) s->adm.release(it->second.off, it->second.bytes);
+  if (it->second.state == RTS_REQ_QUEUED) s->adm.remove(ticket);
   s->reqs.erase(it);
   s->cv_loop.notify_one();
 }
@@ -404,16 +368,20 @@ void rts_server_stats(const rts_server *s, uint64_t *submitted, uint64_t *comple
   if (completed) *completed = s->completed;
   if (batches) *batches = s->batches;
   if (max_batch_seen) *max_batch_seen = s->max_batch_seen;
-  if (pool_in_use) *pool_in_use = s->pool.in_use();
-  if (pool_peak) *pool_peak = s->pool.peak();
+  if (pool_in_use) *pool_in_use = s->adm.pool().in_use();
+  if (pool_peak) *pool_peak = s->adm.pool().peak();
 }
 
 hipError_t rts_server_stop(rts_server *s) {
   if (!s) return hipErrorInvalidValue;
   if (s->running) {
-    s->running = false;
+    {
+      std::lock_guard<std::mutex> lock(s->mu);
+      s->running = false;
+    }
     s->cv_loop.notify_all();
     if (s->loop.joinable()) s->loop.join();
+    s->cv_done.notify_all();
   }
   for (Worker &w : s->workers)
     if (w.stream) (void)hipStreamSynchronize(w.stream);
@@ -430,7 +398,7 @@ void rts_server_destroy(rts_server *s) {
     if (w.done) (void)hipEventDestroy(w.done);
     if (w.stream) (void)hipStreamDestroy(w.stream);
   }
-  s->pool.destroy();
+  s->region.destroy();
   delete s;
 }
 
@@ -439,7 +407,7 @@ hipError_t rts_server_replay(rts_server *s, const int *table_ids, const rts_quer
                              unsigned long long *counts, double *elapsed_s) {
   if (!s || !s->running || nq < 0 || (nq && (!table_ids || !queries || !arrival_ns || !latency_ns)))
     return hipErrorInvalidValue;
-  const uint64_t done0 = s->completed;
+  ReplayState rs;  // this replay's requests only: an earlier failure does not end it early
   const uint64_t t0 = now_ns();
   hipError_t err = hipSuccess;
   int sent = 0;  // requests actually submitted: the wait below counts only those
@@ -452,7 +420,7 @@ hipError_t rts_server_replay(rts_server *s, const int *table_ids, const rts_quer
       }
       hipError_t e;
       while ((e = submit_at(s, table_ids[i], &queries[i], due, &latency_ns[i], counts ? &counts[i] : nullptr,
-                            true, nullptr)) == hipErrorNotReady)
+                            &rs, nullptr)) == hipErrorNotReady)
         std::this_thread::sleep_for(std::chrono::microseconds(20));
       if (e != hipSuccess) {
         err = e;
@@ -463,11 +431,30 @@ hipError_t rts_server_replay(rts_server *s, const int *table_ids, const rts_quer
   });
   submitter.join();
   {
+    // every submitted request must be done or failed: until then the loop may still write
+    // latency_ns / counts, which belong to the caller
     std::unique_lock<std::mutex> lock(s->mu);
-    s->cv_done.wait(lock, [&] { return s->completed - done0 >= (uint64_t)sent || s->loop_error != hipSuccess; });
+    s->cv_done.wait(lock, [&] { return rs.finished >= (uint64_t)sent || !s->running; });
+    if (rs.finished < (uint64_t)sent) {
+      // the server was stopped under this replay: its loop no longer runs, so drop the
+      // replay's remaining requests (their batches drained first) so that nothing refers
+      // to this call's state after it returns
+      for (Worker &w : s->workers) (void)hipStreamSynchronize(w.stream);
+      for (auto it = s->reqs.begin(); it != s->reqs.end();) {
+        Request &r = it->second;
+        if (r.replay != &rs) {
+          ++it;
+          continue;
+        }
+        if (r.state == RTS_REQ_QUEUED) s->adm.remove(it->first);
+        if (r.off != SIZE_MAX) s->adm.release(r.off, r.bytes);
+        it = s->reqs.erase(it);
+      }
+      if (rs.error == hipSuccess) rs.error = hipErrorNotReady;
+    }
   }
   if (elapsed_s) *elapsed_s = (now_ns() - t0) * 1e-9;
-  return err != hipSuccess ? err : s->loop_error;
+  return err != hipSuccess ? err : rs.error;
 }
 
 }  // extern "C"

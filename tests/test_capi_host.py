@@ -181,15 +181,18 @@ def test_entry_points_reject_bad_arguments_before_any_device_work():
 def test_batch_default_config_is_the_documented_shape():
     """The default launch shape (include/rts_batch.h, used when a plan gets no config):
     2048-word tiles, up to 32 queries per work item, 4 workgroups per CU, locality order,
-    256-thread workgroups; shapes outside the kernels' instantiations are refused."""
+    256-thread workgroups, the plan's own index set, no direct form for indexed columns;
+    shapes outside the kernels' instantiations, negative weights and bin counts no index set
+    has are refused."""
     cfg = R.BatchConfig()
     R.lib().rts_batch_default_config(ctypes.byref(cfg))
-    assert cfg.astuple() == (2048, 32, 4, 1, 256)
+    assert cfg.astuple() == (2048, 32, 4, 1, 256, 0, 0)
     L = R.lib()
     plan = ctypes.c_void_p()
     cols = (ctypes.c_void_p * 1)(None)
     # invalid shapes are refused before any device work (valid ones would allocate)
-    for bad in ((2048, 32, 4, 1, 128), (4096, 32, 4, 1, 256), (2048, 65, 4, 1, 256)):
+    for bad in ((2048, 32, 4, 1, 128, 0, 0), (4096, 32, 4, 1, 256, 0, 0), (2048, 65, 4, 1, 256, 0, 0),
+                (2048, 32, 4, 1, 256, 0, -1), (2048, 32, 4, 1, 256, 256, 0)):
         c = R.BatchConfig(*bad)
         assert L.rts_batch_plan_create(ctypes.byref(plan), cols, None, 1, 1000, 4, ctypes.byref(c),
                                        None) == HIP_ERROR_INVALID_VALUE, bad

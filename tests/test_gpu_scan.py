@@ -320,3 +320,57 @@ def test_reference_wrappers_from_several_threads(eng):
     for t in ts:
         t.join()
     assert not errors, errors[:3]
+
+
+def test_scratch_is_bounded_per_thread_and_freed_with_the_thread(eng):
+    """Counted scans on many streams keep at most 16 scratch buffers in the calling thread
+    (the oldest is freed first) with every count right; threads that exit release theirs
+    (the device's free memory comes back after 40 short-lived counting threads)."""
+    import threading
+    L = eng.lib()
+    n = 1 << 16
+    x = np.random.default_rng(9).integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    d = eng.DeviceBuffer.from_numpy(x)
+    out = eng.DeviceBuffer(4 * (n // 32))
+    cnt = eng.DeviceBuffer(8)
+    errors = []
+
+    def counted_scans(nstreams):
+        streams = [eng.Stream() for _ in range(nstreams)]
+        for i, s in enumerate(streams):
+            lo, hi = 1000 * i, (1 << 31) + 77 * i
+            eng.scan_range(d, n, lo, hi, out, cnt, stream=s)
+            s.synchronize()
+            if int(cnt.to_numpy(np.uint64, 1)[0]) != int(((x >= lo) & (x <= hi)).sum()):
+                errors.append(i)
+        held = L.rts_thread_scratch_buffers()
+        for s in streams:
+            s.destroy()
+        return held
+
+    def worker(res):
+        try:
+            res.append(counted_scans(3))
+        except Exception as e:  # noqa: BLE001
+            errors.append(repr(e))
+
+    held = []
+    t = threading.Thread(target=worker, args=(held,))
+    t.start()
+    t.join()
+    assert held == [3]
+    eng.synchronize()
+    free0 = eng.device_free_bytes()
+    for _ in range(40):
+        t = threading.Thread(target=worker, args=(held,))
+        t.start()
+        t.join()
+    eng.synchronize()
+    # 40 threads x 3 buffers x 256 KB would be 30 MB if nothing were freed
+    assert eng.device_free_bytes() >= free0 - (8 << 20)
+    res = []
+    t = threading.Thread(target=lambda: res.append(counted_scans(24)))
+    t.start()
+    t.join()
+    assert res == [16]
+    assert not errors, errors[:3]

@@ -405,3 +405,54 @@ def test_bitmap_digest_properties():
         one[i] = bm[i]
         parts += O.bitmap_digest(one, n) - dz
     assert (parts + dz) % 2 ** 64 == d
+
+
+@pytest.mark.parametrize("n", [1, 31, 33, 1000, 70001])
+def test_conjunction_digest_matches_raw_scan_entry(n):
+    """orc_conjunction_digest (no bitmap) equals the reference checker's path: raw_scan_entry
+    per column into zeroed bitmaps with the AND merge (bindex.cpp:2265-2289), closed ranges
+    written in the query-file grammar (bt lo-1 hi+1, ge lo, le hi), then orc_bitmap_digest
+    and the popcount; with empty ranges, full ranges and 1..4 columns."""
+    rng = np.random.default_rng(n + 3)
+    cols = [rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32),
+            rng.integers(0, 50, n, dtype=np.uint32),
+            (rng.normal(2.0 ** 31, 2.0 ** 28, n).clip(0, MAXU)).astype(np.uint32),
+            rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)]
+    cases = [[(0, MAXU)], [(10, 20)], [(5, 4)], [(0, 0)], [(MAXU, MAXU)]]
+    for _ in range(12):
+        k = int(rng.integers(1, 5))
+        rs = []
+        for c in range(k):
+            a, b = sorted(int(v) for v in rng.integers(0, 2 ** 32, 2))
+            if c == 1:
+                a, b = sorted(int(v) for v in rng.integers(0, 50, 2))
+            rs.append((a, b))
+        cases.append(rs)
+    for rs in cases:
+        use = cols[:len(rs)]
+        W = O.words_for(n)
+        bms = [np.zeros(W, np.uint32) for _ in use]
+        for c, (col, (lo, hi)) in enumerate(zip(use, rs)):
+            if lo > hi:
+                pass  # no row: the zeroed bitmap stays empty
+            elif lo == 0 and hi == MAXU:
+                O.raw_scan_entry(col, O.GE, [0], bitmap=bms[c], merge=bms[0])
+                continue
+            elif lo == 0:
+                O.raw_scan_entry(col, O.LE, [hi], bitmap=bms[c], merge=bms[0])
+                continue
+            elif hi == MAXU:
+                O.raw_scan_entry(col, O.GE, [lo], bitmap=bms[c], merge=bms[0])
+                continue
+            else:
+                O.raw_scan_entry(col, O.BT, [lo - 1], [hi + 1], bitmap=bms[c], merge=bms[0])
+                continue
+            if c > 0:
+                bms[0] &= bms[c]
+        want = (O.bitmap_digest(bms[0], n), O.popcount_rows(bms[0], n))
+        assert O.conjunction_digest(use, rs, threads=3) == want, rs
+        # and the closed form
+        m = np.ones(n, bool)
+        for col, (lo, hi) in zip(use, rs):
+            m &= (col >= lo) & (col <= hi)
+        assert want[1] == int(m.sum())
