@@ -78,25 +78,142 @@ def reference_predicate(lo, hi):
     return oracle.BT, lo - 1, hi + 1
 
 
-def cpu_baseline(x, preds, threads, budget_s=1.5, max_preds=24):
-    """The oracle's port of raw_scan (bindex.cpp:1851-1892), word-parallel over `threads`
-    host threads, on the first predicates of the workload until ~budget_s of wall time."""
+def host_info():
+    """The host the CPU baselines run on: CPU model, the SIMD level of the timed port, the
+    host's CPUs and the CPUs this process may run on (its affinity mask)."""
     import oracle
 
-    n = x.size
+    model, flags = "unknown", set()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name") and model == "unknown":
+                    model = line.split(":", 1)[1].strip()
+                elif line.startswith("flags") and not flags:
+                    flags = set(line.split(":", 1)[1].split())
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count() or 1
+    return {"cpu_model": model, "isa": "avx512f" if "avx512f" in flags else "avx2" if "avx2" in flags else "sse",
+            "port_simd": "avx2" if oracle.simd_level() == 2 else "scalar",
+            "host_cpus": os.cpu_count(), "usable_cpus": usable}
+
+
+def cpu_threads():
+    return host_info()["usable_cpus"]
+
+
+class quiet_stdout:
+    """Sends file descriptor 1 to /dev/null while the reference's compiled code runs (it
+    prints timings); the benchmark's one JSON line stays the only stdout output."""
+
+    def __enter__(self):
+        sys.stdout.flush()
+        self.saved = os.dup(1)
+        self.null = os.open(os.devnull, os.O_WRONLY)
+        os.dup2(self.null, 1)
+        return self
+
+    def __exit__(self, *exc):
+        os.dup2(self.saved, 1)
+        os.close(self.saved)
+        os.close(self.null)
+        return False
+
+
+def port_time(columns, queries, threads, budget_s, max_queries):
+    """The oracle's vectorised port of the checker's conjunction (orc_conjunction_scan: raw_scan
+    + AND merge, bindex.cpp:1851-1892 and 2265-2289) on `threads` host threads over the
+    queries (each a list of (column, lo, hi)) until ~budget_s of wall time.  Returns
+    (queries done, seconds, popcounts)."""
+    import oracle
+
+    n = columns[0].size
+    bm = np.empty((n + 31) // 32, np.uint32)
     done, t_total, counts = 0, 0.0, []
-    for lo, hi in preds[:max_preds]:
-        bm = np.zeros((n + 31) // 32, np.uint32)
-        op, t1, t2 = reference_predicate(lo, hi)
+    for q in queries[:max_queries]:
+        cols = [columns[c] for c, _, _ in q]
+        rs = [(lo, hi) for _, lo, hi in q]
         t0 = time.perf_counter()
-        oracle.raw_scan(x, op, t1, t2, bitmap=bm, threads=threads)
+        oracle.conjunction_scan(cols, rs, threads=threads, bitmap=bm)
         t_total += time.perf_counter() - t0
         counts.append(oracle.popcount_rows(bm, n))
         done += 1
         if t_total >= budget_s:
             break
-    return {"rows_per_s": done * n / t_total, "preds": done, "wall_s": t_total,
-            "counts": counts, "reference": reference_raw_scan(x, preds, counts)}
+    return done, t_total, counts
+
+
+def cpu_baseline(x, order, threads, budget_s=8.0):
+    """The timed CPU port on the headline workload: the predicates in `order` (the GPU
+    loop's first, then the rest of the 1000) until ~budget_s of wall time."""
+    done, t_total, counts = port_time([x], [[(0, lo, hi)] for lo, hi in order], threads, budget_s, len(order))
+    return {"rows_per_s": done * x.size / t_total, "preds": done, "wall_s": t_total,
+            "counts": counts, "order": order[:done], "reference": reference_raw_scan(x, order, counts)}
+
+
+class RefBindexSet:
+    """The reference's own CPU BinDex, compiled from bindex.cpp at N = 1e8 into oracle/_ref,
+    over host columns of 1e8 rows (built concurrently, one host thread per column, each
+    build using the reference's own threads).  Queries are timed over the reference driver's
+    timed region (timer slot 11, bindex.cpp:2190-2262: the BinDex scan of every column, then
+    the AND merge on THREAD_NUM threads)."""
+
+    def __init__(self, columns):
+        import threading
+
+        import oracle
+
+        self.lib = oracle.RefBindexLib(N_ROWS)  # FileNotFoundError when not built
+        self.columns = columns
+        self.b = [None] * len(columns)
+        t0 = time.perf_counter()
+
+        def build(c):
+            self.b[c] = self.lib.bindex(columns[c])
+
+        with quiet_stdout():
+            ts = [threading.Thread(target=build, args=(c,)) for c in range(len(columns))]
+            for t in ts:
+                t.start()
+            for t in ts:
+                t.join()
+        self.build_s = time.perf_counter() - t0
+        self.mins = [int(c.min()) for c in columns]
+        self.maxs = [int(c.max()) for c in columns]
+        W = (N_ROWS + 31) // 32
+        self.bitmaps = []
+        for _ in range(4):
+            raw = np.empty(W + 8, np.uint32)
+            off = (-raw.ctypes.data % 32) // 4
+            self.bitmaps.append(raw[off:off + W])
+
+    def usable(self, q):
+        """The reference asserts (and, built without asserts, reads out of bounds) for bounds
+        outside the data (bindex.cpp:1521-1523, 1583): time only ranges inside it."""
+        return all(self.mins[c] < lo and hi < self.maxs[c] for c, lo, hi in q)
+
+    def time_queries(self, queries, budget_s, max_queries):
+        import oracle
+
+        done, ms, agree = 0, 0.0, 0
+        with quiet_stdout():
+            for q in queries:
+                if done >= max_queries or ms / 1e3 >= budget_s:
+                    break
+                if not self.usable(q):
+                    continue
+                preds = [reference_predicate(lo, hi) for _, lo, hi in q]
+                ms += self.lib.query_slot11([self.b[c] for c, _, _ in q], preds, self.bitmaps[:len(q)])
+                want = oracle.conjunction_scan([self.columns[c] for c, _, _ in q], [(lo, hi) for _, lo, hi in q])
+                agree += int(np.array_equal(oracle.mask_tail(self.bitmaps[0], N_ROWS), want))
+                done += 1
+        return {"queries": done, "ms_per_query": ms / max(done, 1), "queries_per_s": done / (ms / 1e3) if ms else 0.0,
+                "threads": self.lib.threads, "kind": "reference",
+                "agree_with_raw_scan": agree}
 
 
 def reference_raw_scan(x, preds, port_counts, budget_s=1.0, max_preds=4):
@@ -161,9 +278,14 @@ def bench_scan(args, R):
     counts = R.DeviceBuffer(8 * max(steps, 1))
     stream = R.Stream()
 
+    # the predicate of step i: the steps stride through all 1000 predicates (20 steps take
+    # every 50th), so any step count samples the whole selectivity range
+    def pred(i):
+        return preds[(i * len(preds)) // steps % len(preds)] if steps < len(preds) else preds[i % len(preds)]
+
     # headline: kernel (a) alone, one predicate per launch, bitmap output
     def scan(i):
-        lo, hi = preds[i % len(preds)]
+        lo, hi = pred(i)
         R.scan_range(dx, n, lo, hi, out, None, stream)
 
     ms = timed_loop(R, stream, steps, warmup, scan)
@@ -173,7 +295,7 @@ def bench_scan(args, R):
 
     # same predicates with the popcount (a one-workgroup finalize launch per query)
     def scan_count(i):
-        lo, hi = preds[i % len(preds)]
+        lo, hi = pred(i)
         R.scan_range(dx, n, lo, hi, out, _Slot(counts, 8 * (i % max(steps, 1))), stream)
 
     ms_count = timed_loop(R, stream, steps, warmup, scan_count)
@@ -188,20 +310,40 @@ def bench_scan(args, R):
     build_ms = e0.elapsed_ms(e1)
     bcounts = R.DeviceBuffer(8 * max(steps, 1))
 
-    def bquery(i):
-        lo, hi = preds[i % len(preds)]
-        R.bindex_query([ix], [lo], [hi], out, _Slot(bcounts, 8 * (i % max(steps, 1))), stream)
+    def bquery_with(index):
+        def bquery(i):
+            lo, hi = pred(i)
+            R.bindex_query([index], [lo], [hi], out, _Slot(bcounts, 8 * (i % max(steps, 1))), stream)
+        return bquery
 
-    ms_b = timed_loop(R, stream, steps, warmup, bquery)
+    ms_b = timed_loop(R, stream, steps, warmup, bquery_with(ix))
     b_counts = bcounts.to_numpy(np.uint64, steps)
     bq_ok = bool(np.array_equal(b_counts, gpu_counts))
     # algorithmic bytes of one BinDex query: two filter vectors + the two boundary bins'
     # refine lists (8 B per entry) + the bitmap write
-    b_alg = 3 * W * 4 + 2 * (ix.refine_entries() / 128.0) * 8
+    b_alg = 3 * W * 4 + 2 * (ix.refine_entries() / ix.bins) * 8
+    # the same queries over a 512-bin index of the column (lists a quarter as long)
+    ix512 = R.BinDexIndex(dx, n, stream, bins=512)
+    ms_b512 = timed_loop(R, stream, steps, warmup, bquery_with(ix512))
+    b512_ok = bool(np.array_equal(bcounts.to_numpy(np.uint64, steps), gpu_counts))
+    b512_alg = 3 * W * 4 + 2 * (ix512.refine_entries() / ix512.bins) * 8
+    ix512.free()
 
-    cores = min(16, os.cpu_count() or 1)
-    cpu = cpu_baseline(x, preds[:steps] if steps else preds, cores)
-    ok = all(int(gpu_counts[i]) == c for i, c in enumerate(cpu["counts"]) if i < steps)
+    cores = cpu_threads()
+    first = [pred(i) for i in range(min(steps, len(preds)))]
+    seen = set(first)
+    cpu = cpu_baseline(x, first + [p for p in preds if p not in seen], cores)
+    # the port's counts against the GPU's, for the predicates both ran
+    gpu_by_pred = {pred(i): int(gpu_counts[i]) for i in range(steps)}
+    ok = all(gpu_by_pred.get(p, c) == c for p, c in zip(cpu["order"], cpu["counts"]))
+    # the reference's own CPU BinDex on the same column and predicates (slot-11 region)
+    try:
+        ref = RefBindexSet([x])
+        ref_b = ref.time_queries([[(0, lo, hi)] for lo, hi in cpu["order"]], 10.0, 200)
+        ref_b["build_s"] = ref.build_s
+        del ref
+    except (FileNotFoundError, OSError) as e:
+        ref_b = {"unavailable": str(e)}
 
     return {
         "metric": METRIC,
@@ -237,22 +379,31 @@ def bench_scan(args, R):
             "unit": "rows/s",
             "cores": cores,
             "kind": "port",
-            "sample": "oracle raw_scan port (bindex.cpp:1851-1892), %d threads, first %d of "
-                      "the 1000 predicates over the full 1e8-row column (%.2f s wall)"
-                      % (cores, cpu["preds"], cpu["wall_s"]),
+            "sample": "oracle's vectorised raw_scan port (orc_conjunction_scan, %s; "
+                      "bindex.cpp:1851-1892), %d threads, %d of the 1000 predicates (striding "
+                      "through all of them) over the full 1e8-row column (%.2f s wall)"
+                      % (host_info()["port_simd"], cores, cpu["preds"], cpu["wall_s"]),
+            "host": host_info(),
         },
         "extra": {
             "scan_with_popcount_ms_per_step": ms_count / steps,
             "bindex_build_ms": build_ms,
             "bindex_query_ms_per_step": ms_b / steps,
             "bindex_query_rows_per_s": n * steps / (ms_b / 1e3),
-            "bindex_query_alg_GBps": b_alg / (ms_b / 1e3 / steps) / 1e9,
+            "bindex_query_roofline": {"bound": "hbm", "alg_bytes": b_alg,
+                                      "achieved_GBps": b_alg / (ms_b / 1e3 / steps) / 1e9,
+                                      "frac": b_alg / (ms_b / 1e3 / steps) / 1e9 / HBM_PEAK_GBS},
+            "bindex_query_512bins_ms_per_step": ms_b512 / steps,
+            "bindex_query_512bins_roofline": {"alg_bytes": b512_alg,
+                                              "frac": b512_alg / (ms_b512 / 1e3 / steps) / 1e9 / HBM_PEAK_GBS},
             "bindex_device_bytes": ix.device_bytes(),
-            # the reference's compiled raw_scan itself on this host, beside the port above
+            # the reference's own compiled code on this host, beside the port above: its
+            # raw_scan (one thread, as it runs) and its CPU BinDex query (slot-11 region)
             "cpu_reference_raw_scan": cpu["reference"],
+            "cpu_reference_bindex": ref_b,
         },
         "parity": {"checked_predicates": len(cpu["counts"]), "popcounts_match_oracle": bool(ok),
-                   "bindex_counts_match_scan": bq_ok},
+                   "bindex_counts_match_scan": bq_ok, "bindex_512bins_counts_match_scan": b512_ok},
         "setup_s": round(time.time() - t0, 2),
     }
 
@@ -282,6 +433,26 @@ class Table:
         rng = np.random.default_rng(seed)
         pick = rng.integers(0, n, 1_000_000)
         self.samples = [np.sort(c[pick]) for c in self.cols]  # for quantile endpoints
+        self.R = R
+        self.sets = {}
+        self._ref = None
+
+    def index_set(self, bins):
+        """The table's columns indexed with `bins` engine bins (built on first use)."""
+        if bins == 128:
+            return self.idx
+        if bins not in self.sets:
+            self.sets[bins] = [self.R.BinDexIndex(d, self.n, bins=bins) for d in self.dcols]
+        return self.sets[bins]
+
+    def ref(self):
+        """The reference's own CPU BinDex over the table's columns (None if not built)."""
+        if self._ref is None:
+            try:
+                self._ref = RefBindexSet(self.cols)
+            except (FileNotFoundError, OSError) as e:
+                self._ref = str(e)
+        return self._ref if not isinstance(self._ref, str) else None
 
     def range_for(self, c, sel, rng):
         """Closed range on column c whose selectivity is about `sel`: endpoints at sample
@@ -359,6 +530,28 @@ def bench_conj(R, table, nq=30, seed=33):
         entry["direct_alg_GBps"] = (k * 4 * n + 4 * W) / (entry["direct_us_per_query"] * 1e-6) / 1e9
         entry["counts_agree"] = counts["direct"] == counts["bindex"] == counts["unfused"]
         entry["realised_selectivity_median"] = float(np.median(counts["direct"])) / n
+        # rooflines: the direct kernel reads every column once and writes the bitmap; the
+        # BinDex kernel reads two filter vectors and two bins' list slices per column
+        dir_bytes = k * 4 * n + 4 * W
+        bx_bytes = float(np.mean([sum(2 * 4 * W + 2 * table.idx[c].refine_entries() / table.idx[c].bins * 8
+                                      for c in cs) + 4 * W for cs, _ in qs]))
+        entry["roofline"] = {
+            "bound": "hbm", "peak_GBps": HBM_PEAK_GBS,
+            "direct": {"alg_bytes": dir_bytes,
+                       "frac": dir_bytes / (entry["direct_us_per_query"] * 1e-6) / 1e9 / HBM_PEAK_GBS},
+            "bindex": {"alg_bytes": bx_bytes,
+                       "frac": bx_bytes / (entry["bindex_us_per_query"] * 1e-6) / 1e9 / HBM_PEAK_GBS}}
+        # CPU baselines on the same queries: the vectorised port (all usable host threads)
+        # and the reference's own CPU BinDex (slot-11 region, its THREAD_NUM threads)
+        cq = [[(int(c), lo, hi) for c, (lo, hi) in zip(cs, rr)] for cs, rr in qs]
+        threads = cpu_threads()
+        done, secs, pc = port_time(table.cols, cq, threads, 3.0, 12)
+        entry["cpu_baseline"] = {
+            "port": {"us_per_query": secs / done * 1e6, "queries_per_s": done / secs, "cores": threads,
+                     "kind": "port", "queries": done, "counts_match_gpu": pc == counts["direct"][:done]}}
+        ref = table.ref()
+        if ref is not None:
+            entry["cpu_baseline"]["reference_bindex"] = ref.time_queries(cq, 3.0, 12)
         res["cols%d" % k] = entry
     # bitmaps, not only counts: one query per column count through the three paths, compared
     # on the device by digest (each path's parity with the oracle is pinned by the tests)
@@ -373,6 +566,61 @@ def bench_conj(R, table, nq=30, seed=33):
         ok = ok and digests[0] == digests[1] == digests[2]
     res["bitmaps_agree"] = ok
     return res
+
+
+def config1_predicates(n_pred=64, seed=20240101):
+    """Config 1's 64 closed ranges: width = selectivity x 2^32 with the selectivity cycling
+    through {0.001, 0.01, 0.1, 0.5, 0.9}, lo uniform on [0, 2^32 - width] (hi never clips)."""
+    rng = np.random.default_rng(seed)
+    sels = [0.001, 0.01, 0.1, 0.5, 0.9]
+    out = []
+    for i in range(n_pred):
+        width = int(round(sels[i % 5] * 2.0 ** 32))
+        lo = int(rng.integers(0, 2 ** 32 - width + 1))
+        out.append((lo, lo + width - 1))
+    return out
+
+
+def bench_config1(R, n=1 << 20, reps=20):
+    """Config 1: one 2^20-row uniform column, 64 closed ranges, one query per launch with the
+    popcount (bitmap + count); every bitmap checked against the oracle's raw_scan."""
+    import oracle
+
+    x = R.generate_column(R.DIST_UNIFORM, 1, 0, n)
+    dx = R.DeviceBuffer.from_numpy(x)
+    W = (n + 31) // 32
+    preds = config1_predicates()
+    stream = R.Stream()
+    out = R.DeviceBuffer(4 * W * len(preds))
+    cnt = R.DeviceBuffer(8 * len(preds))
+
+    def launch(i):
+        j = i % len(preds)
+        lo, hi = preds[j]
+        R.scan_range(dx, n, lo, hi, _Slot(out, 4 * W * j), _Slot(cnt, 8 * j), stream)
+
+    steps = reps * len(preds)
+    ms = timed_loop(R, stream, steps, len(preds), launch)
+    got = out.to_numpy(np.uint32, W * len(preds)).reshape(len(preds), W)
+    counts = cnt.to_numpy(np.uint64, len(preds))
+    ok = True
+    for j, (lo, hi) in enumerate(preds):
+        op, t1, t2 = reference_predicate(lo, hi)
+        truth = oracle.raw_scan(x, op, t1, t2, threads=cpu_threads())
+        ok = ok and bool(np.array_equal(got[j], truth)) and int(counts[j]) == oracle.popcount_rows(truth, n)
+    us = ms * 1e3 / steps
+    alg = 4 * n + 4 * W
+    threads = cpu_threads()
+    done, secs, _ = port_time([x], [[(0, lo, hi)] for lo, hi in preds] * 20, threads, 2.0, 64 * 20)
+    ref = reference_raw_scan(x, preds, [], budget_s=1.0, max_preds=64)
+    out.free()
+    dx.free()
+    return {"rows": n, "predicates": len(preds), "us_per_query": us, "rows_per_s": n / (us * 1e-6),
+            "roofline": {"bound": "hbm (launch-bound at this size)", "alg_bytes": alg,
+                         "frac": alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS},
+            "cpu_baseline": {"port": {"rows_per_s": done * n / secs, "cores": threads, "kind": "port"},
+                             "reference_raw_scan": ref},
+            "bitmaps_and_counts_match_oracle": ok}
 
 
 def bench_aux(R, n=N_ROWS, snap_rows=1 << 24, reps=3):
@@ -475,34 +723,43 @@ def batch_queries(table, nq, seed, direct_share):
     return qs
 
 
-def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cache=None,
-                config=None):
-    """Config 4: nq mixed queries in one persistent hipGraph-captured launch, launch shape
-    from the autotuner, against the plain one-query-per-launch paths (direct conjunction,
-    and BinDex) on the same GPU."""
+def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=None):
+    """Config 4: nq mixed queries in one persistent hipGraph-captured launch.  The launch
+    shape comes from the autotuner in two stages: tile / group / workgroup shape on the
+    table's 128-bin indexes, then the bin count (index sets of 128, 256 and 512 bins) and the
+    planner's direct weight, which decides per query-column between the direct and the BinDex
+    form.  Compared with the plain one-query-per-launch paths (direct conjunction, and BinDex)
+    on the same GPU and with the CPU baselines on a sample of the same queries."""
     n = table.n
     W = (n + 31) // 32
     stride = (W + 3) // 4 * 4
-    qs = batch_queries(table, nq, seed, direct_share)
+    qs = batch_queries(table, nq, seed, 0.0)
     stream = R.Stream()
     out = R.DeviceBuffer(4 * stride * nq)
     counts = R.DeviceBuffer(8 * nq)
     plan = R.BatchPlan(table.dcols, table.idx, n, nq, None, stream)
+    for bins in (256, 512):
+        plan.add_indexes(table.index_set(bins))
     plan.set_queries(qs, stream)
+    t0 = time.perf_counter()
     if config:  # fixed launch shape (profiling runs)
         cands = [R.batch_config(*config)]
+        chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache, stream=stream)
+        stages = [len(cands)]
     else:
-        cands = [R.batch_config(t, g, 4, o, nt) for t in (512, 1024, 2048) for g in (16, 32, 64)
-                 for o in (0, 1) for nt in (256, 512)]
-    t0 = time.perf_counter()
-    chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache,
-                                    stream=stream)
+        shapes = [R.batch_config(t, g, 4, 1, nt) for t in (1024, 2048) for g in (32, 64) for nt in (256, 512)]
+        shape, _ = plan.autotune(out, stride, shapes, reps=2, stream=stream)
+        s = shape.astuple()[:5]
+        plans = [R.batch_config(*s, bins=b, direct_weight=w) for b in (128, 256, 512) for w in (0, 40, 80)]
+        chosen, tune_ms = plan.autotune(out, stride, plans, reps=2, cache_path=tune_cache, stream=stream)
+        stages = [len(shapes), len(plans)]
     tune_s = time.perf_counter() - t0
     plan.graph_launch(out, stride, counts, stream)  # capture + first run
     stream.synchronize()
     ms = timed_loop(R, stream, steps, 1,
                     lambda i: plan.graph_launch(out, stride, counts, stream))
     batch_counts = counts.to_numpy(np.uint64, nq)
+    st = plan.stats()
 
     # plain one-query-per-launch paths over the same queries
     one = R.DeviceBuffer(4 * W)
@@ -536,12 +793,22 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
         b = np.empty(W, np.uint32)
         R.check(R.lib().rts_memcpy_dtoh(b.ctypes.data, out.ptr + 4 * stride * i, 4 * W), "dtoh")
         same = same and bool(np.array_equal(a, b))
-    st = plan.stats()
     qps = nq * steps / (ms / 1e3)
     qps_plain = nq / (ms_plain / 1e3)
     qps_plain_b = nq / (ms_plain_b / 1e3)
     out_bytes = nq * W * 4
+    floor_bytes = out_bytes + len(table.dcols) * 4 * n  # every bitmap written + the table read once
     plan.free()
+    # CPU baselines on a sample of the same queries
+    cq = [[(c, lo, hi) for c, lo, hi, _ in q] for q in qs]
+    threads = cpu_threads()
+    done, secs, pc = port_time(table.cols, cq, threads, 4.0, 32)
+    cpu = {"port": {"queries_per_s": done / secs, "cores": threads, "kind": "port", "queries": done,
+                    "counts_match_gpu": [int(v) for v in batch_counts[:done]] == pc}}
+    ref = table.ref()
+    if ref is not None:
+        cpu["reference_bindex"] = ref.time_queries(cq, 4.0, 32)
+    total_cols = st["bindex_cols"] + st["direct_cols"]
     return {
         "queries": nq,
         "batch_ms": ms / steps,
@@ -551,10 +818,16 @@ def bench_batch(R, table, nq=4096, seed=44, direct_share=0.02, steps=5, tune_cac
         "speedup_vs_plain_direct": qps / qps_plain,
         "speedup_vs_plain_bindex": qps / qps_plain_b,
         "bitmap_write_GBps": out_bytes / (ms / steps / 1e3) / 1e9,
+        "roofline": {"bound": "hbm", "alg_bytes": floor_bytes,
+                     "achieved_GBps": floor_bytes / (ms / steps / 1e3) / 1e9,
+                     "frac": floor_bytes / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS,
+                     "traffic": load_traffic("batch_4096q_1e8x4_traffic.txt")},
+        "cpu_baseline": cpu,
         "autotune": {"chosen": list(chosen.astuple()), "best_ms": tune_ms, "search_s": tune_s,
-                     "candidates": len(cands)},
+                     "candidates": stages},
+        "chosen_bins": chosen.bins or 128,
         "planner": st,
-        "direct_share": direct_share,
+        "planner_direct_share": st["direct_cols"] / max(1, total_cols),
         "counts_match_plain": bool(np.array_equal(batch_counts, plain_counts)) and
                               bool(np.array_equal(batch_counts, plain_b_counts)),
         "sampled_bitmaps_match_plain": same,
@@ -587,16 +860,18 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
     the result pool's free HBM; continuous batching over 3 streams."""
     t_setup = time.time()
     n1 = 1 << 26
+    import oracle
+
     t1_cols = [R.generate_column(0, seed, c, n1) for c in range(2)]
     t1_d = [R.DeviceBuffer.from_numpy(c) for c in t1_cols]
     t1_ix = [R.BinDexIndex(d, n1) for d in t1_d]
-    t2_d, t2_ix, t2_samples = [], [], []
+    t2_d, t2_ix, t2_samples, t2_cols = [], [], [], []
     rng = np.random.default_rng(seed)
-    for c, (k, p0, p1) in enumerate(TABLE_KINDS):  # one column at a time bounds host memory
+    for c, (k, p0, p1) in enumerate(TABLE_KINDS):
         col = R.generate_column(k, seed + 1, c, n2, p0, p1)
         t2_samples.append(np.sort(col[rng.integers(0, n2, 1_000_000)]))
         t2_d.append(R.DeviceBuffer.from_numpy(col))
-        del col
+        t2_cols.append(col)  # kept on the host for the CPU baseline and the oracle checks
         t2_ix.append(R.BinDexIndex(t2_d[-1], n2))
     t1_samples = [np.sort(c[rng.integers(0, n1, 1_000_000)]) for c in t1_cols]
     setup_s = time.time() - t_setup
@@ -620,18 +895,15 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
         lat, cnt, elapsed = srv.replay(tids, qs, arrivals)
         st1 = srv.stats()
         lat_ms = lat.astype(np.float64) / 1e6
-        # bit-exactness: counts of a sample of queries against the one-query-per-launch path
+        # bit-exactness: counts of a sample of queries against the oracle's conjunction
         check = list(range(0, nq, max(1, nq // 16)))
-        one1 = R.DeviceBuffer(4 * ((n1 + 31) // 32))
-        one2 = R.DeviceBuffer(4 * ((n2 + 31) // 32))
-        c = R.DeviceBuffer(8)
         ok = True
         for i in check:
-            q = qs[i]
-            ix = t1_ix if tids[i] == tid1 else t2_ix
-            R.bindex_query([ix[cc] for cc, _, _, _ in q], [lo for _, lo, _, _ in q],
-                           [hi for _, _, hi, _ in q], one1 if tids[i] == tid1 else one2, c)
-            ok = ok and int(c.to_numpy(np.uint64, 1)[0]) == int(cnt[i])
+            cols = t1_cols if tids[i] == tid1 else t2_cols
+            _d, c = oracle.conjunction_digest([cols[cc] for cc, _, _, _ in qs[i]],
+                                              [(lo, hi) for _, lo, hi, _ in qs[i]], threads=cpu_threads())
+            ok = ok and c == int(cnt[i])
+        out_bytes = sum((n1 if t == tid1 else n2) + 31 for t in tids) // 32 * 4
         out["rates"].append({
             "offered_qps": rate, "queries": nq,
             "throughput_qps": nq / elapsed,
@@ -639,9 +911,28 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
             "mean_ms": float(lat_ms.mean()),
             "batches": st1["batches"] - st0["batches"], "max_batch": st1["max_batch_seen"],
             "pool_peak_gb": st1["pool_peak"] / 2 ** 30,
-            "sampled_counts_match": bool(ok), "checked": len(check),
+            "bitmap_write_GBps": out_bytes / elapsed / 1e9,
+            "roofline_frac": out_bytes / elapsed / 1e9 / HBM_PEAK_GBS,
+            "sampled_counts_match_oracle": bool(ok), "checked": len(check),
         })
+        last_qs, last_tids = qs, tids
     srv.destroy()
+    # CPU baseline: the vectorised port, one query after another over both tables (the
+    # reference has no request loop; its driver runs one query at a time)
+    threads = cpu_threads()
+    t_cpu, done = 0.0, 0
+    for i in range(min(32, len(last_qs))):
+        cols = t1_cols if last_tids[i] == tid1 else t2_cols
+        q = [(cc, lo, hi) for cc, lo, hi, _ in last_qs[i]]
+        d1, s1, _ = port_time(cols, [q], threads, 1e9, 1)
+        t_cpu += s1
+        done += d1
+        if t_cpu > 10.0:
+            break
+    out["cpu_baseline"] = {"queries_per_s": done / t_cpu, "cores": threads, "kind": "port", "queries": done,
+                           "sample": "vectorised port, the last rate's first queries in arrival order"}
+    sustained = [r for r in out["rates"] if r["throughput_qps"] >= 0.95 * r["offered_qps"]]
+    out["max_sustained_offered_qps"] = max((r["offered_qps"] for r in sustained), default=0)
     return out
 
 
@@ -663,7 +954,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="scan",
-                    choices=["scan", "copycal", "conj", "batch", "serve", "aux"])
+                    choices=["scan", "copycal", "conj", "batch", "serve", "aux", "config1"])
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
@@ -686,6 +977,8 @@ def main():
         res = bench_copycal(args, R)
     elif args.workload == "aux":
         res = {"aux": bench_aux(R)}
+    elif args.workload == "config1":
+        res = {"config1": bench_config1(R)}
     elif args.workload == "serve":
         cfg = tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config else None
         res = {"serve": bench_serve(R, n2=args.serve_rows, batch_config=cfg)}
@@ -700,6 +993,10 @@ def main():
     else:
         res = bench_scan(args, R)
         if not args.no_extra:
+            try:
+                res["extra"]["config1"] = bench_config1(R)
+            except Exception as e:  # noqa: BLE001
+                res["extra"]["config1_error"] = "%s: %s" % (type(e).__name__, e)
             # config 3 and config 4 legs; a failure here is reported, not fatal to the line
             try:
                 table = Table(R)

@@ -100,10 +100,10 @@ void rts_batch_plan_stats(const rts_batch_plan *plan, int *bindex_cols, int *dir
 /* RTS_EVAL_AUTO query-columns of the current queries that the cost model sent to the
  * direct form although their column has an index (-1 for a NULL plan). */
 int rts_batch_plan_auto_direct(const rts_batch_plan *plan);
-/* 64-bit digest of the current query mix: a histogram of query shapes (column count, and
- * per column its evaluation mode and a log2 bucket of the fraction of the column's 128 areas
- * the range touches), each cell's share rounded to 1/64.  Batches drawn from one distribution get
- * the same digest; the autotuner's cache is keyed by it. */
+/* 64-bit digest of the current query mix: mean columns per query (in quarters) and the
+ * shares of query-columns per evaluation mode and per range width (empty, 1-8, 9-32 or more
+ * of the column's 128 areas), in sixteenths.  Batches drawn from one distribution get the
+ * same digest; the autotuner's cache is keyed by it. */
 unsigned long long rts_batch_plan_mix_digest(const rts_batch_plan *plan);
 /* Work-item groups the current queries were planned into, and (cfg non-NULL) the launch
  * shape they were planned for; -1 for a NULL plan. */

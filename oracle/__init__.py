@@ -91,6 +91,10 @@ def lib():
         L.orc_conjunction_digest.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _u32p, _u32p,
                                              ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.orc_conjunction_digest.restype = None
+        L.orc_conjunction_scan.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, _u32p, _u32p, _u32p,
+                                           ctypes.c_int]
+        L.orc_conjunction_scan.restype = None
+        L.orc_simd_level.restype = ctypes.c_int
         L.orc_bitmap_digest.restype = ctypes.c_uint64
         L.orc_normal_encode.argtypes = [_u32p, ctypes.c_int, ctypes.c_uint32, ctypes.c_uint32,
                                         _u32p, _i32p, _u32p, _u32p, _u32p]
@@ -173,6 +177,25 @@ def conjunction_digest(columns, ranges, threads=None):
     lib().orc_conjunction_digest(ptrs, len(cols), n, lo, hi, threads or min(16, os.cpu_count() or 1),
                                  ctypes.byref(d), ctypes.byref(c))
     return d.value, c.value
+
+
+def conjunction_scan(columns, ranges, threads=None, bitmap=None):
+    """Bitmap of the AND over `columns` of closed ranges [(lo, hi), ...]: the vectorised,
+    word-parallel port of the checker's conjunction (orc_conjunction_scan)."""
+    cols = [np.ascontiguousarray(c, dtype=np.uint32) for c in columns]
+    n = cols[0].size
+    ptrs = (ctypes.c_void_p * len(cols))(*[c.ctypes.data for c in cols])
+    lo = np.array([r[0] & 0xFFFFFFFF for r in ranges], np.uint32)
+    hi = np.array([r[1] & 0xFFFFFFFF for r in ranges], np.uint32)
+    if bitmap is None:
+        bitmap = np.empty(max(words_for(n), 1), np.uint32)
+    lib().orc_conjunction_scan(ptrs, len(cols), n, lo, hi, bitmap, threads or min(16, os.cpu_count() or 1))
+    return bitmap
+
+
+def simd_level():
+    """2 when the host runs the AVX2 port, 0 for the scalar one."""
+    return int(lib().orc_simd_level())
 
 
 def compare_bitmap(truth, got, n):
@@ -282,6 +305,10 @@ class RefBindexLib:
                                       ctypes.c_uint32, _u32p]
         L.ref_raw_scan.argtypes = [_u32p, ctypes.c_int, _u32p, ctypes.c_uint32,
                                    ctypes.c_uint32, ctypes.c_int]
+        L.ref_query_slot11.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p,
+                                       ctypes.c_void_p, ctypes.c_void_p]
+        L.ref_query_slot11.restype = ctypes.c_double
+        L.ref_thread_num.restype = ctypes.c_int
         self.L = L
         self.n = L.ref_compiled_n()
         self.K = L.ref_compiled_k()
@@ -294,6 +321,23 @@ class RefBindexLib:
 
     def bindex(self, data):
         return _RefBindex(self, data)
+
+    @property
+    def threads(self):
+        return int(self.L.ref_thread_num())
+
+    def query_slot11(self, bindexes, preds, bitmaps):
+        """The reference driver's timed region (bindex.cpp:2190-2262) for one query: the
+        reference BinDex scan of every column (preds: (op, t1, t2) per column) into
+        bitmaps[c], then the multithreaded AND merge into bitmaps[0].  Returns its wall
+        time in ms.  bitmaps: 32-byte aligned uint32 arrays of words_for(n) words."""
+        k = len(bindexes)
+        hs = (ctypes.c_void_p * k)(*[b.h for b in bindexes])
+        ops = np.array([p[0] for p in preds], np.int32)
+        t1 = np.array([p[1] & 0xFFFFFFFF for p in preds], np.uint32)
+        t2 = np.array([p[2] & 0xFFFFFFFF for p in preds], np.uint32)
+        bms = (ctypes.c_void_p * k)(*[b.ctypes.data for b in bitmaps])
+        return float(self.L.ref_query_slot11(hs, k, ops.ctypes.data, t1.ctypes.data, t2.ctypes.data, bms))
 
 
 class _RefBindex:

@@ -10,6 +10,7 @@
  */
 #include "oracle.h"
 
+#include <immintrin.h>
 #include <omp.h>
 #include <stdlib.h>
 #include <string.h>
@@ -57,6 +58,93 @@ void orc_raw_scan_mt(const uint32_t *raw, int n, uint32_t *bitmap, uint32_t t1, 
       if (raw_hit(raw[i], t1, t2, op)) acc ^= row_bit(i);
     bitmap[w] ^= acc;
   }
+}
+
+/* ---- vectorised port of the checker's conjunction (timed as the CPU baseline) ----------
+ * The same rows as raw_scan_entry per column + the AND merge (bindex.cpp:1851-1892,
+ * 2265-2289) for closed ranges lo[c] <= x <= hi[c], written straight into `bitmap`:
+ * 8 rows per AVX2 compare ((x - lo) ^ 2^31 > (hi - lo) ^ 2^31 signed = outside), the
+ * movemask bits reversed into MSB-first order.  Scalar on hosts without AVX2. */
+static uint8_t rev8_table[256];
+
+static void init_rev8(void) {
+  for (int b = 0; b < 256; b++) {
+    int r = 0;
+    for (int i = 0; i < 8; i++)
+      if (b & (1 << i)) r |= 1 << (7 - i);
+    rev8_table[b] = (uint8_t)r;
+  }
+}
+
+static uint32_t conj_word_scalar(const uint32_t *const *cols, int ncols, const uint32_t *lo,
+                                 const uint32_t *span, int64_t r0, int rows) {
+  uint32_t acc = rows == 32 ? 0xFFFFFFFFu : ~(0xFFFFFFFFu >> rows);
+  for (int c = 0; c < ncols; c++) {
+    uint32_t m = 0;
+    for (int i = 0; i < rows; i++)
+      if (cols[c][r0 + i] - lo[c] <= span[c]) m |= 0x80000000u >> i;
+    acc &= m;
+  }
+  return acc;
+}
+
+__attribute__((target("avx2"))) static void conj_words_avx2(const uint32_t *const *cols, int ncols,
+                                                            const uint32_t *lo, const uint32_t *span,
+                                                            int64_t w0, int64_t w1, uint32_t *bitmap) {
+  const __m256i bias = _mm256_set1_epi32((int)0x80000000u);
+  __m256i vlo[4], vsp[4];
+  for (int c = 0; c < ncols; c++) {
+    vlo[c] = _mm256_set1_epi32((int)lo[c]);
+    vsp[c] = _mm256_xor_si256(_mm256_set1_epi32((int)span[c]), bias);
+  }
+  for (int64_t w = w0; w < w1; w++) {
+    unsigned m[4] = {0xFF, 0xFF, 0xFF, 0xFF};
+    for (int c = 0; c < ncols; c++) {
+      const uint32_t *x = cols[c] + w * 32;
+      for (int g = 0; g < 4; g++) {
+        __m256i v = _mm256_loadu_si256((const __m256i *)(x + 8 * g));
+        v = _mm256_xor_si256(_mm256_sub_epi32(v, vlo[c]), bias);
+        const int miss = _mm256_movemask_ps(_mm256_castsi256_ps(_mm256_cmpgt_epi32(v, vsp[c])));
+        m[g] &= ~(unsigned)miss & 0xFFu;
+      }
+    }
+    bitmap[w] = ((uint32_t)rev8_table[m[0]] << 24) | ((uint32_t)rev8_table[m[1]] << 16) |
+                ((uint32_t)rev8_table[m[2]] << 8) | (uint32_t)rev8_table[m[3]];
+  }
+}
+
+int orc_simd_level(void) {
+  __builtin_cpu_init();
+  return __builtin_cpu_supports("avx2") ? 2 : 0;
+}
+
+void orc_conjunction_scan(const uint32_t *const *cols, int ncols, int64_t n, const uint32_t *lo,
+                          const uint32_t *hi, uint32_t *bitmap, int threads) {
+  if (n <= 0 || ncols < 1 || ncols > 4) return;
+  const int64_t words = (n + 31) / 32, full = n / 32;
+  uint32_t span[4];
+  for (int c = 0; c < ncols; c++) {
+    if (lo[c] > hi[c]) {
+      memset(bitmap, 0, words * sizeof(uint32_t));
+      return;
+    }
+    span[c] = hi[c] - lo[c];
+  }
+  if (!rev8_table[1]) init_rev8();
+  const int avx2 = orc_simd_level() == 2;
+  if (threads < 1) threads = 1;
+  const int64_t chunk = 4096;
+  const int64_t nchunks = (full + chunk - 1) / chunk;
+#pragma omp parallel for num_threads(threads) schedule(static)
+  for (int64_t k = 0; k < nchunks; k++) {
+    const int64_t w0 = k * chunk, w1 = w0 + chunk < full ? w0 + chunk : full;
+    if (avx2) {
+      conj_words_avx2(cols, ncols, lo, span, w0, w1, bitmap);
+    } else {
+      for (int64_t w = w0; w < w1; w++) bitmap[w] = conj_word_scalar(cols, ncols, lo, span, w * 32, 32);
+    }
+  }
+  if (full < words) bitmap[full] = conj_word_scalar(cols, ncols, lo, span, full * 32, (int)(n - full * 32));
 }
 
 /* raw_scan_entry, bindex.cpp:1894-1957: for every target of a query line, swap reversed

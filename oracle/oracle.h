@@ -19,6 +19,8 @@
  *                                 closed form fv[k] = (x < start[k+1]) in numpy.
  *   orc_in_which_area/_block,     compiled reference on the same BinDex.
  *   orc_on_which_pos
+ *   orc_conjunction_scan          orc_raw_scan_entry (bt / le / ge per column, AND merge) on
+ *                                 ragged row counts, 1..4 columns, empty and full ranges.
  *   orc_conjunction_digest        orc_raw_scan_entry (bt / le / ge per column, AND merge) and
  *                                 orc_bitmap_digest on the same inputs, including empty
  *                                 ranges and ragged row counts.
@@ -98,6 +100,14 @@ int orc_refine_eq_area(const orc_bindex *const *bxs, const uint32_t *const *raws
  * mix64((i << 32) | word_i), bits past row n cleared, mix64(x) = splitmix64 finaliser of
  * x ^ 0x9E3779B97F4A7C15.  Not a reference function: a checksum shared by both sides. */
 uint64_t orc_bitmap_digest(const uint32_t *bitmap, int64_t n);
+
+/* The checker's conjunction (raw_scan_entry per column + AND merge, bindex.cpp:1851-1892,
+ * 2265-2289) for closed ranges lo[c] <= x <= hi[c] over 1..4 columns, vectorised (AVX2 when
+ * the host has it: orc_simd_level() == 2) and word-parallel: the CPU baseline's timed port.
+ * Writes all ceil(n/32) words, tail bits 0.  Pinned against orc_raw_scan_entry on ragged n. */
+void orc_conjunction_scan(const uint32_t *const *cols, int ncols, int64_t n, const uint32_t *lo,
+                          const uint32_t *hi, uint32_t *bitmap, int threads);
+int orc_simd_level(void);
 
 /* Digest (orc_bitmap_digest) and popcount of the AND over ncols columns of n rows of the
  * closed ranges lo[c] <= x <= hi[c] (lo > hi: no row), computed word by word without the

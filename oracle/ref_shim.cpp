@@ -11,9 +11,11 @@
 #include "bindex.cpp"
 #undef main
 
+#include <chrono>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <thread>
 
 namespace {
 BITS *aligned_words(int words) {
@@ -91,5 +93,48 @@ void ref_raw_scan(const uint32_t *raw, int n, uint32_t *bitmap, uint32_t t1, uin
   b.length = n;
   raw_scan(&b, bitmap, t1, t2, (OPERATOR)op, const_cast<CODE *>(raw));
 }
+
+}  // extern "C"
+
+extern "C" {
+
+// The reference driver's timed region for one query (timer slot 11, bindex.cpp:2190-2262):
+// bindex_scan_{lt,le,gt,ge,bt,eq} of every column into that column's bitmap, then the AND
+// merge of columns 1.. into column 0's bitmap by THREAD_NUM refine_result_bitmap threads
+// over word ranges, as the driver splits them.  Returns the region's wall time in ms.
+// bitmaps[c] must hold bits_num_needed(N) words, 32-byte aligned.
+double ref_query_slot11(void **bxs, int ncols, const int *ops, const uint32_t *t1, const uint32_t *t2,
+                        uint32_t **bitmaps) {
+  const auto t0 = std::chrono::steady_clock::now();
+  for (int c = 0; c < ncols; c++) {
+    BinDex *b = (BinDex *)bxs[c];
+    switch (ops[c]) {
+      case LT: bindex_scan_lt(b, bitmaps[c], t1[c]); break;
+      case LE: bindex_scan_le(b, bitmaps[c], t1[c]); break;
+      case GT: bindex_scan_gt(b, bitmaps[c], t1[c]); break;
+      case GE: bindex_scan_ge(b, bitmaps[c], t1[c]); break;
+      case BT: bindex_scan_bt(b, bitmaps[c], t1[c], t2[c]); break;
+      default: bindex_scan_eq(b, bitmaps[c], t1[c]); break;
+    }
+  }
+  const int max_idx = (N + CODEWIDTH - 1) / CODEWIDTH;
+  const int stride = (max_idx + THREAD_NUM - 1) / THREAD_NUM;
+  for (int c = 1; c < ncols; c++) {
+    std::thread threads[THREAD_NUM];
+    int start = 0, t = 0;
+    while (start < max_idx && t < THREAD_NUM) {
+      const int end = start + stride > max_idx ? max_idx : start + stride;
+      threads[t] = std::thread(refine_result_bitmap, bitmaps[0], bitmaps[c], start, end, t);
+      start += stride;
+      t++;
+    }
+    for (int i = 0; i < t; i++) threads[i].join();
+  }
+  const auto t1_ = std::chrono::steady_clock::now();
+  fflush(stdout);
+  return std::chrono::duration<double, std::milli>(t1_ - t0).count();
+}
+
+int ref_thread_num() { return THREAD_NUM; }
 
 }  // extern "C"

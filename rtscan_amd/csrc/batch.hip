@@ -1090,45 +1090,40 @@ int rts_batch_plan_auto_direct(const rts_batch_plan *p) { return p ? p->stats_au
 
 unsigned long long rts_batch_plan_mix_digest(const rts_batch_plan *p) {
   if (!p) return 0;
-  std::map<std::vector<int>, int> cells;
+  // summary statistics of the batch, each coarsely quantised: mean columns per query (in
+  // quarters), the share of query-columns per evaluation mode and per width class (in
+  // sixteenths); width classes: empty, 1-8 of the 128 areas, 9-32, more
+  int64_t qcols = 0, mode[3] = {0, 0, 0}, width[4] = {0, 0, 0, 0};
   for (const rts_query_spec &q : p->specs) {
-    std::vector<std::array<int, 2>> cols;
     for (int c = 0; c < q.ncols && c < kBatchMaxCols; c++) {
+      qcols++;
+      mode[q.mode[c] >= 0 && q.mode[c] < 3 ? q.mode[c] : 0]++;
       const int col = q.column[c];
       const CODE lo = q.lo[c], hi = q.hi[c];
-      double frac = 0;
       const BinDex *b = col >= 0 && col < p->ncols ? p->index[col] : nullptr;
+      double frac;
       if (lo > hi) {
         frac = 0;
       } else if (b && b->dev) {
         // areas (the reference's 128, the same for every bin count) the range touches
         const rts_bindex_dev *d = b->dev;
         const int a0 = lo <= d->starts[0] ? 0 : bin_of(d->starts, kAreas, lo);
-        const int a1 = hi == 0xFFFFFFFFu || hi + 1u > d->starts[kAreas - 1] ? kAreas - 1 : bin_of(d->starts, kAreas, hi);
+        const int a1 = hi >= d->starts[kAreas - 1] ? kAreas - 1 : bin_of(d->starts, kAreas, hi);
         frac = (double)(a1 - a0 + 1) / kAreas;
       } else {
         frac = ((double)hi - (double)lo + 1.0) / 4294967296.0;
       }
-      const int bucket = frac <= 0 ? -64 : std::max(-32, (int)std::floor(std::log2(frac)));
-      cols.push_back({q.mode[c], bucket});
+      width[frac <= 0 ? 0 : frac <= 8.0 / kAreas ? 1 : frac <= 32.0 / kAreas ? 2 : 3]++;
     }
-    std::sort(cols.begin(), cols.end());
-    std::vector<int> key{q.ncols};
-    for (const auto &a : cols) key.insert(key.end(), a.begin(), a.end());
-    cells[key]++;
   }
+  const size_t nq = p->specs.size();
+  std::vector<int> feat;
+  feat.push_back(nq ? (int)std::lround(4.0 * qcols / nq) : 0);
+  for (int64_t m : mode) feat.push_back(qcols ? (int)std::lround(16.0 * m / qcols) : 0);
+  for (int64_t w : width) feat.push_back(qcols ? (int)std::lround(16.0 * w / qcols) : 0);
   uint64_t h = 0xCBF29CE484222325ull;
-  auto mixin = [&h](int v) {
+  for (int v : feat)
     for (int i = 0; i < 4; i++) h = (h ^ (uint8_t)(v >> (8 * i))) * 0x100000001B3ull;
-  };
-  const int nq = std::max<int>(1, (int)p->specs.size());
-  for (const auto &cell : cells) {
-    const int share = (int)std::lround(64.0 * cell.second / nq);
-    if (!share) continue;
-    for (int v : cell.first) mixin(v);
-    mixin(-1000);
-    mixin(share);
-  }
   return (unsigned long long)h;
 }
 

@@ -215,3 +215,50 @@ def test_reference_shaped_caller_compiles_and_runs_on_host(tmp_path):
     r = subprocess.run([exe, "host"], capture_output=True, text=True, timeout=60)
     assert r.returncode == 0, r.stderr
     assert r.stdout.startswith("ok ")
+
+
+def test_snapshot_host_section_damage_is_refused_before_device_work(tmp_path):
+    """rts_bindex_load checks the header and the host arrays' structure (with their digest)
+    before it touches the device: each of these single-field damages is refused with
+    hipErrorInvalidValue, even without a GPU."""
+    import snapshot_format as S
+    from rtscan_amd.bindex_struct import BinDex
+    L = R.lib()
+    f0, a0 = S.synthetic()
+    cases = {}
+    f = dict(f0); f["magic"] = b"RTSBDX\x00\x01"; cases["old format"] = (f, a0)
+    f = dict(f0); f["bins"] = 200; cases["bins not 128 * sub"] = (f, a0)
+    f = dict(f0); f["fv_stride"] += 32; cases["stride not 64-word aligned"] = (f, a0)
+    f = dict(f0); f["nb"] = f0["n"] + 1; cases["more list entries than rows"] = (f, a0)
+    f = dict(f0); f["ntiles"] += 1; cases["tile count"] = (f, a0)
+    a = {k: v.copy() for k, v in a0.items()}; a["area_start"][5] += 1; cases["areas do not tile"] = (f0, a)
+    a = {k: v.copy() for k, v in a0.items()}; a["area_len"][0] = 0; a["area_len"][1] += 4000
+    cases["empty area"] = (f0, a)
+    a = {k: v.copy() for k, v in a0.items()}; a["starts"][7] = 0; a["estarts"][7] = 0
+    cases["starts decrease"] = (f0, a)
+    a = {k: v.copy() for k, v in a0.items()}; a["boff"][3] = 9; cases["list offsets past nb"] = (f0, a)
+    f = dict(f0); f["nblocks"] += 1
+    a = {k: v.copy() for k, v in a0.items()}
+    a["block_first"] = np.zeros(f["nblocks"], np.uint32); a["block_last"] = a["block_first"].copy()
+    cases["block count"] = (f, a)
+    for name, (f, a) in cases.items():
+        p = tmp_path / "x.rtsbdx"
+        p.write_bytes(S.build(f, a))
+        bx = BinDex()
+        assert L.rts_bindex_load(ctypes.byref(bx), str(p).encode(), None) == HIP_ERROR_INVALID_VALUE, name
+    # the undamaged file passes these checks (without a GPU it then fails at the first
+    # device call, with another error)
+    if L.rts_device_count() < 1:
+        for sub in (1, 4):
+            f, a = S.synthetic(sub=sub)
+            p = tmp_path / "ok.rtsbdx"
+            p.write_bytes(S.build(f, a))
+            bx = BinDex()
+            assert L.rts_bindex_load(ctypes.byref(bx), str(p).encode(), None) not in (0, HIP_ERROR_INVALID_VALUE)
+    # a damaged host digest alone is refused too
+    good = bytearray(S.build(f0, a0))
+    good[S.HEADER.size + 3] ^= 1
+    p = tmp_path / "d.rtsbdx"
+    p.write_bytes(bytes(good))
+    bx = BinDex()
+    assert L.rts_bindex_load(ctypes.byref(bx), str(p).encode(), None) == HIP_ERROR_INVALID_VALUE

@@ -252,7 +252,7 @@ def test_pos_block_views_match_oracle(eng, n):
         ix.host_blocks()
         ix.host_blocks()  # idempotent
         nums = ob.block_nums()
-        starts = ob.area_start()
+        starts = ob.area_start
         for k in list(range(0, 128, 9)) + [127]:
             a = ix.s.areas[k].contents
             assert a.blockNum == nums[k], (name, k)
