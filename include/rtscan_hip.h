@@ -126,7 +126,8 @@ const char *rts_last_error(void);
 /* Scratch buffers (count partials, checker parts) the calling thread holds: one per
  * (kind, device, stream) it has used, at most 16 (the oldest is freed first); a thread's
  * buffers are freed when it exits.  Counted and checking calls allocate on their first use
- * of a stream, so that first call is not for graph capture. */
+ * of a stream, so that first call is not for graph capture (rts_conjunction_scan_noalloc
+ * is). */
 int rts_thread_scratch_buffers(void);
 
 /* ------------------------------------------------------------ additive: scan kernels */
@@ -152,6 +153,19 @@ hipError_t rts_scan_range_xor(const CODE *column, uint64_t n, CODE lo, CODE hi, 
 hipError_t rts_conjunction_scan(const CODE *const *columns, const CODE *lo, const CODE *hi,
                                 int ncols, uint64_t n, BITS *out, unsigned long long *count,
                                 hipStream_t stream);
+
+/* No counterpart in the reference.  For callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc): rts_conjunction_scan with
+ * the count's per-workgroup partials in a fixed 256 KB device array of this library (one per
+ * GPU, held until the process ends) instead of per-thread scratch, so no call allocates.
+ * n <= RTS_NOALLOC_MAX_ROWS (2^40 rows; more is refused with hipErrorInvalidValue).  Every
+ * call on one GPU shares that GPU's array and each counted call rewrites it before reading
+ * it: calls that overlap on one GPU (from several streams or host threads) race, so order
+ * them.  Asynchronous on `stream`, like rts_conjunction_scan. */
+#define RTS_NOALLOC_MAX_ROWS (1ull << 40)
+hipError_t rts_conjunction_scan_noalloc(const CODE *const *columns, const CODE *lo, const CODE *hi,
+                                        int ncols, uint64_t n, BITS *out, unsigned long long *count,
+                                        hipStream_t stream);
 
 /* Kernel (c), bitmap form: out = AND of `nbitmaps` n-row device bitmaps, popcount of the
  * first n rows into `count` (may be NULL), one pass.  Replaces the chain of

@@ -70,6 +70,7 @@ SIGNATURES = {
     "rts_scan_range": (_err, [_vp, _u64, _u32, _u32, _vp, _vp, _vp]),
     "rts_scan_range_xor": (_err, [_vp, _u64, _u32, _u32, _vp, _vp]),
     "rts_conjunction_scan": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
+    "rts_conjunction_scan_noalloc": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
     "rts_bitmap_and_popcount": (_err, [_vp, _int, _u64, _vp, _vp, _vp]),
     "rts_bitmap_popcount": (_err, [_vp, _u64, _vp, _vp]),
     # BinDex surface

@@ -3,6 +3,7 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <atomic>
 #include <map>
 #include <mutex>
 #include <thread>
@@ -17,7 +18,7 @@ namespace rts {
 
 hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE *hi, int ncols,
                             uint64_t n, BITS *out, unsigned long long *count, int mode,
-                            hipStream_t s);
+                            hipStream_t s, uint32_t *partials = nullptr);
 hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS *out,
                                unsigned long long *count, hipStream_t s);
 hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
@@ -37,21 +38,21 @@ hipError_t set_error(hipError_t e, const char *what) {
   return e;
 }
 
+// Per-device caches of this file are fixed arrays (no allocation after load).
+constexpr int kMaxDevices = 64;
+
 int device_cu_count() {
-  static std::mutex mu;
-  static std::vector<int> cache;
+  static std::atomic<int> cache[kMaxDevices];
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess) return 256;
-  std::lock_guard<std::mutex> lock(mu);
-  if ((int)cache.size() <= dev) cache.resize(dev + 1, 0);
-  if (cache[dev] == 0) {
-    int cus = 0;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        cus <= 0)
+  if (dev < 0 || dev >= kMaxDevices) return 256;
+  int cus = cache[dev].load(std::memory_order_relaxed);
+  if (cus == 0) {
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
       cus = 256;
-    cache[dev] = cus;
+    cache[dev].store(cus, std::memory_order_relaxed);
   }
-  return cache[dev];
+  return cus;
 }
 
 // Scratch buffers of the calling host thread, keyed by (kind, device, stream).  Two threads
@@ -289,6 +290,62 @@ hipError_t rts_conjunction_scan(const CODE *const *columns, const CODE *lo, cons
                                 hipStream_t stream) {
   return set_error(scan_conjunction(columns, lo, hi, ncols, n, out, count, kStoreSet, stream),
                    "rts_conjunction_scan");
+}
+
+}  // extern "C"
+
+// The allocation-free conjunction's scratch: the count partials of one counted launch
+// (kMaxCountBlocks 32-bit slots, 256 KB), a file-scope device array, one copy per GPU held
+// until the process ends.  Host code reaches it only through the address
+// hipGetSymbolAddress gives for the current device (the array's name is a host-side
+// stand-in, not a device address), fetched once per device on the first call there.
+namespace {
+
+__device__ __align__(256) uint32_t scratch[rts::kMaxCountBlocks];
+
+std::atomic<uint32_t *> g_scratch_ptr[rts::kMaxDevices];
+std::mutex g_scratch_mu;
+
+hipError_t noalloc_scratch(uint32_t **out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= rts::kMaxDevices) return hipErrorInvalidDevice;
+  uint32_t *p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+  if (!p) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    p = g_scratch_ptr[dev].load(std::memory_order_acquire);
+    if (!p) {
+      void *q = nullptr;
+      e = hipGetSymbolAddress(&q, HIP_SYMBOL(scratch));  // with `dev` current
+      if (e != hipSuccess) return e;
+      if (!q) return hipErrorInvalidSymbol;
+      p = static_cast<uint32_t *>(q);
+      g_scratch_ptr[dev].store(p, std::memory_order_release);
+    }
+  }
+  *out = p;
+  return hipSuccess;
+}
+
+}  // namespace
+
+extern "C" {
+
+hipError_t rts_conjunction_scan_noalloc(const CODE *const *columns, const CODE *lo, const CODE *hi,
+                                        int ncols, uint64_t n, BITS *out, unsigned long long *count,
+                                        hipStream_t stream) {
+  if (n > RTS_NOALLOC_MAX_ROWS)
+    return set_error(hipErrorInvalidValue, "rts_conjunction_scan_noalloc: more than RTS_NOALLOC_MAX_ROWS (2^40) rows");
+  uint32_t *partials = nullptr;
+  if (count) {
+    // every workgroup of the counted launch writes its own slot before the finalize reads
+    // it: nothing to clear between calls
+    hipError_t e = noalloc_scratch(&partials);
+    if (e != hipSuccess) return set_error(e, "rts_conjunction_scan_noalloc: scratch address");
+  }
+  return set_error(scan_conjunction(columns, lo, hi, ncols, n, out, count, kStoreSet, stream, partials),
+                   "rts_conjunction_scan_noalloc");
 }
 
 hipError_t rts_bitmap_and_popcount(const BITS *const *bitmaps, int nbitmaps, uint64_t n,

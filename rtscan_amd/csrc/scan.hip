@@ -266,9 +266,9 @@ static hipError_t launch_scan_c(const ColPreds<C> &p, uint64_t n, BITS *out, Cou
 
 template <int C, int MODE>
 static hipError_t launch_scan(const ColPreds<C> &p, uint64_t n, BITS *out,
-                              unsigned long long *count, hipStream_t s) {
+                              unsigned long long *count, hipStream_t s, uint32_t *partials) {
   if (count) {
-    CountSink sink{count, stream_count_partials(s)};
+    CountSink sink{count, partials ? partials : stream_count_partials(s)};
     if (!sink.partials) return hipErrorOutOfMemory;
     return launch_scan_c<C, MODE, true>(p, n, out, sink, s);
   }
@@ -289,23 +289,26 @@ static hipError_t launch_empty(uint64_t n, BITS *out, unsigned long long *count,
 
 template <int C>
 static hipError_t scan_group(const CODE *const *cols, const CODE *lo, const CODE *hi, uint64_t n,
-                             BITS *out, unsigned long long *count, int mode, hipStream_t s) {
+                             BITS *out, unsigned long long *count, int mode, hipStream_t s,
+                             uint32_t *partials) {
   ColPreds<C> p;
   for (int c = 0; c < C; c++) {
     p.col[c] = cols[c];
     p.lo[c] = lo[c];
     p.span[c] = hi[c] - lo[c];
   }
-  if (mode == kStoreSet) return launch_scan<C, kStoreSet>(p, n, out, count, s);
-  if (mode == kStoreXor) return launch_scan<C, kStoreXor>(p, n, out, count, s);
-  return launch_scan<C, kStoreAnd>(p, n, out, count, s);
+  if (mode == kStoreSet) return launch_scan<C, kStoreSet>(p, n, out, count, s, partials);
+  if (mode == kStoreXor) return launch_scan<C, kStoreXor>(p, n, out, count, s, partials);
+  return launch_scan<C, kStoreAnd>(p, n, out, count, s, partials);
 }
 
 // Conjunction over any number of columns: groups of up to 4 columns per pass, the first
-// pass in `mode`, later passes AND into the result; the popcount on the last pass.
+// pass in `mode`, later passes AND into the result; the popcount on the last pass, its
+// per-workgroup partials in `partials` (kMaxCountBlocks slots) when given, else in the
+// calling thread's scratch for the stream.
 hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE *hi, int ncols,
                             uint64_t n, BITS *out, unsigned long long *count, int mode,
-                            hipStream_t s) {
+                            hipStream_t s, uint32_t *partials) {
   if (ncols < 1 || !cols || !lo || !hi) return hipErrorInvalidValue;
   if (mode == kStoreXor && ncols > 4) return hipErrorInvalidValue;
   if (n == 0) return count ? launch_empty(0, out, count, kStoreXor, s) : hipSuccess;
@@ -322,10 +325,10 @@ hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE 
     unsigned long long *cnt = last ? count : nullptr;
     hipError_t e;
     switch (g) {
-      case 1: e = scan_group<1>(cols + done, lo + done, hi + done, n, out, cnt, m, s); break;
-      case 2: e = scan_group<2>(cols + done, lo + done, hi + done, n, out, cnt, m, s); break;
-      case 3: e = scan_group<3>(cols + done, lo + done, hi + done, n, out, cnt, m, s); break;
-      default: e = scan_group<4>(cols + done, lo + done, hi + done, n, out, cnt, m, s); break;
+      case 1: e = scan_group<1>(cols + done, lo + done, hi + done, n, out, cnt, m, s, partials); break;
+      case 2: e = scan_group<2>(cols + done, lo + done, hi + done, n, out, cnt, m, s, partials); break;
+      case 3: e = scan_group<3>(cols + done, lo + done, hi + done, n, out, cnt, m, s, partials); break;
+      default: e = scan_group<4>(cols + done, lo + done, hi + done, n, out, cnt, m, s, partials); break;
     }
     if (e != hipSuccess) return e;
     done += g;
