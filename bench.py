@@ -35,6 +35,7 @@ N_ROWS = 100_000_000
 N_PRED = 1000
 DATA_SEED = 2
 PRED_SEED = 22
+CPU_BASELINES = True  # --no-cpu: skip the CPU baselines of the extra legs (profiling runs)
 
 
 def config2_predicates(count=N_PRED, seed=PRED_SEED):
@@ -76,6 +77,11 @@ def reference_predicate(lo, hi):
     if hi == 0xFFFFFFFF:
         return oracle.GE, lo, 0
     return oracle.BT, lo - 1, hi + 1
+
+
+def progress(msg):
+    """One line per finished step of the run, on stderr (stdout carries only the JSON line)."""
+    print("[bench] %s" % msg, file=sys.stderr, flush=True)
 
 
 def host_info():
@@ -157,30 +163,23 @@ def cpu_baseline(x, order, threads, budget_s=8.0):
 
 class RefBindexSet:
     """The reference's own CPU BinDex, compiled from bindex.cpp at N = 1e8 into oracle/_ref,
-    over host columns of 1e8 rows (built concurrently, one host thread per column, each
-    build using the reference's own threads).  Queries are timed over the reference driver's
-    timed region (timer slot 11, bindex.cpp:2190-2262: the BinDex scan of every column, then
-    the AND merge on THREAD_NUM threads)."""
+    over host columns of 1e8 rows, built one column after another as its driver does (its
+    init_bindex keeps the sorted copy in a global, bindex.cpp:598-676, so two builds must
+    not run at once; each build uses the reference's own threads).  Queries are timed over
+    the reference driver's timed region (timer slot 11, bindex.cpp:2190-2262: the BinDex scan
+    of every column, then the AND merge on THREAD_NUM threads)."""
 
     def __init__(self, columns):
-        import threading
-
         import oracle
 
         self.lib = oracle.RefBindexLib(N_ROWS)  # FileNotFoundError when not built
         self.columns = columns
-        self.b = [None] * len(columns)
         t0 = time.perf_counter()
-
-        def build(c):
-            self.b[c] = self.lib.bindex(columns[c])
-
-        with quiet_stdout():
-            ts = [threading.Thread(target=build, args=(c,)) for c in range(len(columns))]
-            for t in ts:
-                t.start()
-            for t in ts:
-                t.join()
+        self.b = []
+        for i, c in enumerate(columns):
+            with quiet_stdout():
+                self.b.append(self.lib.bindex(c))
+            progress("reference CPU BinDex built for column %d of %d" % (i + 1, len(columns)))
         self.build_s = time.perf_counter() - t0
         self.mins = [int(c.min()) for c in columns]
         self.maxs = [int(c.max()) for c in columns]
@@ -289,6 +288,7 @@ def bench_scan(args, R):
         R.scan_range(dx, n, lo, hi, out, None, stream)
 
     ms = timed_loop(R, stream, steps, warmup, scan)
+    progress("headline scan: %.2f us per launch" % (ms * 1e3 / steps))
     per_launch_s = ms / 1e3 / steps
     alg_bytes = n * 4 + W * 4  # column read + bitmap write
     achieved = alg_bytes / per_launch_s / 1e9
@@ -329,6 +329,7 @@ def bench_scan(args, R):
     b512_alg = 3 * W * 4 + 2 * (ix512.refine_entries() / ix512.bins) * 8
     ix512.free()
 
+    progress("BinDex queries timed (128 and 512 bins)")
     cores = cpu_threads()
     first = [pred(i) for i in range(min(steps, len(preds)))]
     seen = set(first)
@@ -337,13 +338,16 @@ def bench_scan(args, R):
     gpu_by_pred = {pred(i): int(gpu_counts[i]) for i in range(steps)}
     ok = all(gpu_by_pred.get(p, c) == c for p, c in zip(cpu["order"], cpu["counts"]))
     # the reference's own CPU BinDex on the same column and predicates (slot-11 region)
-    try:
-        ref = RefBindexSet([x])
-        ref_b = ref.time_queries([[(0, lo, hi)] for lo, hi in cpu["order"]], 10.0, 200)
-        ref_b["build_s"] = ref.build_s
-        del ref
-    except (FileNotFoundError, OSError) as e:
-        ref_b = {"unavailable": str(e)}
+    progress("CPU port timed: %.3g rows/s" % cpu["rows_per_s"])
+    ref_b = {"skipped": "--no-cpu"}
+    if CPU_BASELINES:
+        try:
+            ref = RefBindexSet([x])
+            ref_b = ref.time_queries([[(0, lo, hi)] for lo, hi in cpu["order"]], 10.0, 200)
+            ref_b["build_s"] = ref.build_s
+            del ref
+        except (FileNotFoundError, OSError) as e:
+            ref_b = {"unavailable": str(e)}
 
     return {
         "metric": METRIC,
@@ -447,6 +451,8 @@ class Table:
 
     def ref(self):
         """The reference's own CPU BinDex over the table's columns (None if not built)."""
+        if not CPU_BASELINES:
+            return None
         if self._ref is None:
             try:
                 self._ref = RefBindexSet(self.cols)
@@ -545,7 +551,7 @@ def bench_conj(R, table, nq=30, seed=33):
         # and the reference's own CPU BinDex (slot-11 region, its THREAD_NUM threads)
         cq = [[(int(c), lo, hi) for c, (lo, hi) in zip(cs, rr)] for cs, rr in qs]
         threads = cpu_threads()
-        done, secs, pc = port_time(table.cols, cq, threads, 3.0, 12)
+        done, secs, pc = port_time(table.cols, cq, threads, 3.0, 12 if CPU_BASELINES else 1)
         entry["cpu_baseline"] = {
             "port": {"us_per_query": secs / done * 1e6, "queries_per_s": done / secs, "cores": threads,
                      "kind": "port", "queries": done, "counts_match_gpu": pc == counts["direct"][:done]}}
@@ -553,6 +559,7 @@ def bench_conj(R, table, nq=30, seed=33):
         if ref is not None:
             entry["cpu_baseline"]["reference_bindex"] = ref.time_queries(cq, 3.0, 12)
         res["cols%d" % k] = entry
+        progress("conjunctions of %d columns timed" % k)
     # bitmaps, not only counts: one query per column count through the three paths, compared
     # on the device by digest (each path's parity with the oracle is pinned by the tests)
     ok = True
@@ -754,12 +761,24 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
         chosen, tune_ms = plan.autotune(out, stride, plans, reps=2, cache_path=tune_cache, stream=stream)
         stages = [len(shapes), len(plans)]
     tune_s = time.perf_counter() - t0
+    progress("batch autotuned: %s" % (chosen.astuple(),))
     plan.graph_launch(out, stride, counts, stream)  # capture + first run
     stream.synchronize()
     ms = timed_loop(R, stream, steps, 1,
                     lambda i: plan.graph_launch(out, stride, counts, stream))
     batch_counts = counts.to_numpy(np.uint64, nq)
     st = plan.stats()
+    # host planning time (bound lookups, cost model, grouping, upload), which a served batch
+    # pays before its launch: the whole batch and a 256-query serving batch
+    plan_ms = {}
+    for m in (nq, 256):
+        t = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            plan.set_queries(qs[:m], stream)
+            t.append((time.perf_counter() - t0) * 1e3)
+        plan_ms[str(m)] = float(np.median(t))
+    plan.set_queries(qs, stream)
 
     # plain one-query-per-launch paths over the same queries
     one = R.DeviceBuffer(4 * W)
@@ -774,6 +793,7 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
         R.conjunction_scan([table.dcols[c] for c, _, _, _ in q], [lo for _, lo, _, _ in q],
                            [hi for _, _, hi, _ in q], n, one, Slot(i), stream)
 
+    progress("batch timed: %.2f ms per launch" % (ms / steps))
     ms_plain = timed_loop(R, stream, nq, 8, plain_direct)
     plain_counts = c1.to_numpy(np.uint64, nq)
 
@@ -802,7 +822,7 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
     # CPU baselines on a sample of the same queries
     cq = [[(c, lo, hi) for c, lo, hi, _ in q] for q in qs]
     threads = cpu_threads()
-    done, secs, pc = port_time(table.cols, cq, threads, 4.0, 32)
+    done, secs, pc = port_time(table.cols, cq, threads, 4.0, 32 if CPU_BASELINES else 1)
     cpu = {"port": {"queries_per_s": done / secs, "cores": threads, "kind": "port", "queries": done,
                     "counts_match_gpu": [int(v) for v in batch_counts[:done]] == pc}}
     ref = table.ref()
@@ -827,6 +847,7 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
                      "candidates": stages},
         "chosen_bins": chosen.bins or 128,
         "planner": st,
+        "host_planning_ms": plan_ms,
         "planner_direct_share": st["direct_cols"] / max(1, total_cols),
         "counts_match_plain": bool(np.array_equal(batch_counts, plain_counts)) and
                               bool(np.array_equal(batch_counts, plain_b_counts)),
@@ -882,6 +903,7 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
     tid1 = srv.add_table(t1_d, t1_ix, n1)
     tid2 = srv.add_table(t2_d, t2_ix, n2)
     srv.start()
+    progress("served tables ready (%.1f s)" % setup_s)
     out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb,
            "batch_config": list(cfg.batch.astuple()), "rates": []}
     for rate, secs in zip(rates, seconds):
@@ -916,12 +938,13 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
             "sampled_counts_match_oracle": bool(ok), "checked": len(check),
         })
         last_qs, last_tids = qs, tids
+        progress("served %d q/s offered: %.0f q/s" % (rate, nq / elapsed))
     srv.destroy()
     # CPU baseline: the vectorised port, one query after another over both tables (the
     # reference has no request loop; its driver runs one query at a time)
     threads = cpu_threads()
     t_cpu, done = 0.0, 0
-    for i in range(min(32, len(last_qs))):
+    for i in range(min(32 if CPU_BASELINES else 1, len(last_qs))):
         cols = t1_cols if last_tids[i] == tid1 else t2_cols
         q = [(cc, lo, hi) for cc, lo, hi, _ in last_qs[i]]
         d1, s1, _ = port_time(cols, [q], threads, 1e9, 1)
@@ -934,6 +957,30 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
     sustained = [r for r in out["rates"] if r["throughput_qps"] >= 0.95 * r["offered_qps"]]
     out["max_sustained_offered_qps"] = max((r["offered_qps"] for r in sustained), default=0)
     return out
+
+
+def bench_readprobe(R, nbytes=4_000_000_000, reps=5):
+    """The device's streaming-read ceiling with the scan kernels' load pattern (rts_probe_read:
+    non-temporal uint4 per lane), swept over grid sizes and loads in flight; the best sweep
+    point is the practical ceiling the scan's bandwidth is set against."""
+    buf = R.DeviceBuffer(nbytes)
+    R.check(R.lib().rts_memset(buf.ptr, 1, nbytes), "memset")
+    sink = R.DeviceBuffer(16)
+    stream = R.Stream()
+    best, points = None, []
+    for bpc in (4, 8, 16, 32):
+        for unroll in (1, 2, 4, 8):
+            def launch(i):
+                R.check(R.lib().rts_probe_read(buf.ptr, nbytes, bpc, unroll, sink.ptr, stream.handle),
+                        "rts_probe_read")
+            ms = timed_loop(R, stream, reps, 1, launch) / reps
+            gbps = nbytes / (ms * 1e-3) / 1e9
+            points.append([bpc, unroll, round(gbps, 1)])
+            if best is None or gbps > best[2]:
+                best = [bpc, unroll, gbps]
+    buf.free()
+    return {"bytes": nbytes, "best_GBps": best[2], "best_blocks_per_cu": best[0], "best_unroll": best[1],
+            "frac_of_8TBps": best[2] / HBM_PEAK_GBS, "sweep": points}
 
 
 def bench_copycal(args, R):
@@ -954,7 +1001,7 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="scan",
-                    choices=["scan", "copycal", "conj", "batch", "serve", "aux", "config1"])
+                    choices=["scan", "copycal", "conj", "batch", "serve", "aux", "config1", "readprobe"])
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
@@ -962,7 +1009,11 @@ def main():
                          "batch workload (skips the autotuner) or the serve workload's batches")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction, batched and serving legs of the default run")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baselines of the conj / batch / serve legs (profiling runs)")
     args = ap.parse_args()
+    global CPU_BASELINES
+    CPU_BASELINES = not args.no_cpu
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
               "(row sharding across GPUs is described in DESIGN.md, not run)" % args.gpus,
@@ -979,6 +1030,8 @@ def main():
         res = {"aux": bench_aux(R)}
     elif args.workload == "config1":
         res = {"config1": bench_config1(R)}
+    elif args.workload == "readprobe":
+        res = {"readprobe": bench_readprobe(R)}
     elif args.workload == "serve":
         cfg = tuple(int(v) for v in args.batch_config.split(",")) if args.batch_config else None
         res = {"serve": bench_serve(R, n2=args.serve_rows, batch_config=cfg)}
@@ -997,6 +1050,10 @@ def main():
                 res["extra"]["config1"] = bench_config1(R)
             except Exception as e:  # noqa: BLE001
                 res["extra"]["config1_error"] = "%s: %s" % (type(e).__name__, e)
+            try:
+                res["extra"]["read_ceiling"] = bench_readprobe(R)
+            except Exception as e:  # noqa: BLE001
+                res["extra"]["read_ceiling_error"] = "%s: %s" % (type(e).__name__, e)
             # config 3 and config 4 legs; a failure here is reported, not fatal to the line
             try:
                 table = Table(R)

@@ -94,6 +94,8 @@ void raw_scan(struct BinDex *bindex, BITS *bitmap, CODE target1, CODE target2,
 
 /* ---------------------------------------------------------- additive: device helpers */
 
+/* Synchronous helpers: each returns after its work has finished, so work then enqueued on
+ * any stream (non-blocking ones included) sees its result. */
 hipError_t rts_malloc(void **ptr, size_t bytes);
 hipError_t rts_free(void *ptr);
 hipError_t rts_memcpy_htod(void *dst, const void *src, size_t bytes);

@@ -63,6 +63,7 @@ SIGNATURES = {
                                ctypes.POINTER(ctypes.c_size_t)]),
     "rts_last_error": (ctypes.c_char_p, []),
     "rts_thread_scratch_buffers": (_int, []),
+    "rts_probe_read": (_err, [_vp, ctypes.c_size_t, _int, _int, _vp, _vp]),
     "rts_device_mem_info": (_err, [ctypes.POINTER(ctypes.c_size_t), ctypes.POINTER(ctypes.c_size_t)]),
     # scans
     "rts_predicate_interval": (_int, [_int, _u32, _u32, ctypes.POINTER(_u32),

@@ -41,18 +41,6 @@ struct QueryArgs {
   BoundDev hi[C];  // L(b + 1)
 };
 
-// Rows of bound b's refine list in tile t with value in [thr, thr + span] (a bound: value
-// >= thr; a narrow column: its whole range): toggled in the tile's LDS toggles.  The list's
-// tile slice [e0, e1) was read from the directory beforehand.
-__device__ __forceinline__ void mark_removed(const BoundDev &b, uint32_t e0, uint32_t e1, int64_t t,
-                                             uint32_t *tog) {
-  const uint32_t row0 = (uint32_t)(t * kTileRows);
-  for (uint32_t e = e0 + threadIdx.x; e < e1; e += blockDim.x) {
-    const uint2 pv = b.bent[e];
-    if (pv.y - b.thr <= b.span) atomicXor(&tog[(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
-  }
-}
-
 __device__ __forceinline__ uint4 base_words(const BoundDev &b, int64_t w, int64_t words) {
   if (b.base == kBaseZero) return make_uint4(0, 0, 0, 0);
   if (b.base == kBaseOnes) return make_uint4(~0u, ~0u, ~0u, ~0u);
@@ -63,6 +51,8 @@ __device__ __forceinline__ uint4 base_words(const BoundDev &b, int64_t w, int64_
   if (w + 2 < words) v.z = b.fv[w + 2];
   return v;
 }
+
+constexpr int kListU = 2;  // list entries per thread per list per round
 
 template <int C, int MODE, bool COUNT>
 __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t n, int64_t words,
@@ -103,10 +93,36 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
     }
     for (int i = threadIdx.x; i < C * kTileWords; i += kBlock) (&tog[0][0])[i] = 0;
     __syncthreads();
+    // The 2C lists' tile slices: every list's next kListU entries per thread are loaded
+    // before any of them is applied, so one round trip covers all lists (a tile's slice of
+    // one list is about kTileRows / bins entries: one or two rounds in practice).
+    const uint32_t row0 = (uint32_t)(t * kTileRows);
+    for (uint32_t off = 0;; off += kListU * kBlock) {
+      uint2 ent[2 * C][kListU];
+      bool more = false;
 #pragma unroll
-    for (int c = 0; c < C; c++) {
-      mark_removed(q.lo[c], lo_e[c][0], lo_e[c][1], t, tog[c]);
-      mark_removed(q.hi[c], hi_e[c][0], hi_e[c][1], t, tog[c]);
+      for (int l = 0; l < 2 * C; l++) {
+        const BoundDev &b = (l & 1) ? q.hi[l / 2] : q.lo[l / 2];
+        const uint32_t e0 = (l & 1) ? hi_e[l / 2][0] : lo_e[l / 2][0];
+        const uint32_t e1 = (l & 1) ? hi_e[l / 2][1] : lo_e[l / 2][1];
+#pragma unroll
+        for (int u = 0; u < kListU; u++) {
+          const uint32_t e = e0 + off + u * kBlock + threadIdx.x;
+          ent[l][u] = e < e1 ? b.bent[e] : make_uint2(0xFFFFFFFFu, 0);
+        }
+        more = more || e0 + off + kListU * kBlock < e1;  // the same in every thread
+      }
+#pragma unroll
+      for (int l = 0; l < 2 * C; l++) {
+        const BoundDev &b = (l & 1) ? q.hi[l / 2] : q.lo[l / 2];
+#pragma unroll
+        for (int u = 0; u < kListU; u++) {
+          const uint2 pv = ent[l][u];
+          if (pv.x != 0xFFFFFFFFu && pv.y - b.thr <= b.span)
+            atomicXor(&tog[l / 2][(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
+        }
+      }
+      if (!more) break;
     }
     __syncthreads();
     if (w < words) {

@@ -409,10 +409,16 @@ hipError_t rts_memcpy_htod_async(void *dst, const void *src, size_t bytes, hipSt
   return set_error(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream), "rts_memcpy_htod_async");
 }
 hipError_t rts_memcpy_dtod(void *dst, const void *src, size_t bytes) {
-  return set_error(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "rts_memcpy_dtod");
+  hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  return set_error(e, "rts_memcpy_dtod");
 }
 hipError_t rts_memset(void *dst, int v, size_t bytes) {
-  return set_error(hipMemset(dst, v, bytes), "rts_memset");
+  // returns after the memory is set, so work the caller then enqueues on any stream
+  // (including non-blocking ones) sees it
+  hipError_t e = hipMemset(dst, v, bytes);
+  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  return set_error(e, "rts_memset");
 }
 hipError_t rts_device_synchronize(void) {
   return set_error(hipDeviceSynchronize(), "rts_device_synchronize");

@@ -9,7 +9,8 @@
 // row count comes from -n, or from the file size.  Every column gets a BinDex
 // (init_bindex_in_GPU).  Queries: the query file (default test/scan_cmd.txt, as there), one
 // line per column.  Each query runs as one fused sieve + refine + AND launch
-// (rts_bindex_query), or -m batch runs all of them in one batched launch; its bitmap is
+// (rts_bindex_query), or -m batch runs all of them in one batched launch (a query with a
+// comma-separated list of targets on a line runs as XOR scans, see below); its bitmap is
 // copied to page-locked host memory and checked against raw_scan of the same predicates
 // with compare_bitmap, which prints "[CHECK]hit true/total" (and "[ERROR] ..." at the first
 // wrong row).  -e encodes the columns first, as the reference's ENCODE build does
@@ -143,14 +144,22 @@ int main(int argc, char **argv) {
   const int nq = rts_query_file_count(qf);
   printf("BinDex scan... (%d queries, %s)\n", nq, mode.c_str());
 
-  // the targets of (query, column), mapped through encodeQuery with -e (rtscan.cpp:1022)
-  auto target = [&](int q, int c, int which) -> CODE {
+  // target i of (query, column), mapped through encodeQuery with -e (rtscan.cpp:1022)
+  auto target = [&](int q, int c, int which, int i = 0) -> CODE {
     const rts_column_predicate *p = rts_query_file_get(qf, q, c);
-    const CODE t = which == 1 ? p->t1[0] : (p->t2 ? p->t2[0] : 0);
+    const CODE t = which == 1 ? p->t1[i] : (p->t2 ? p->t2[i] : 0);
     if (!encode || which != 1) return t;
     std::string cmd = op_name(p->op);
     return encodeQuery(c, t, cmd);
   };
+  // A line with several comma-separated targets selects, on that column, the rows hit by an
+  // odd number of them: the reference's checker XORs each target's hits into the column's
+  // bitmap (raw_scan_entry, bindex.cpp:1894-1930), the ground truth followed here.  Such
+  // queries run as one XOR scan per target and an AND + popcount over the columns.
+  std::vector<char> has_list(nq, 0);
+  for (int q = 0; q < nq; q++)
+    for (int c = 0; c < ncols; c++)
+      if (rts_query_file_get(qf, q, c)->ntargets != 1) has_list[q] = 1;
   if (encode)
     for (int q = 0; q < nq; q++)
       for (int c = 0; c < ncols; c++) {
@@ -169,10 +178,6 @@ int main(int argc, char **argv) {
     sp.ncols = ncols;
     for (int c = 0; c < ncols; c++) {
       const rts_column_predicate *p = rts_query_file_get(qf, q, c);
-      if (p->ntargets != 1) {
-        fprintf(stderr, "[ERROR] query %d column %d: lists of targets are not supported\n", q, c);
-        return 2;
-      }
       sp.column[c] = c;
       if (!rts_predicate_interval((enum OPERATOR)p->op, target(q, c, 1), target(q, c, 2), &sp.lo[c], &sp.hi[c])) {
         sp.lo[c] = 1;  // empty: lo > hi
@@ -193,24 +198,54 @@ int main(int argc, char **argv) {
   hipStream_t stream = nullptr;
   if (rts_stream_create(&stream) != hipSuccess) return fail("stream");
 
+  // per-column bitmaps of the target-list queries
+  std::vector<BITS *> colbm_dev(ncols, nullptr);
+  bool any_list = false;
+  for (int q = 0; q < nq; q++) any_list = any_list || has_list[q];
+  for (int k = 0; any_list && k < ncols; k++)
+    if (rts_malloc((void **)&colbm_dev[k], stride * 4) != hipSuccess) return fail("allocation");
+  auto list_query = [&](int q, BITS *out, unsigned long long *count) -> hipError_t {
+    for (int c = 0; c < ncols; c++) {
+      const rts_column_predicate *p = rts_query_file_get(qf, q, c);
+      // cleared on the query's stream (a hipMemset on the null stream is not ordered
+      // before work on this non-blocking stream)
+      hipError_t e = hipMemsetAsync(colbm_dev[c], 0, words * 4, stream);
+      for (int i = 0; e == hipSuccess && i < p->ntargets; i++) {
+        CODE lo, hi;
+        if (rts_predicate_interval((enum OPERATOR)p->op, target(q, c, 1, i), target(q, c, 2, i), &lo, &hi))
+          e = rts_scan_range_xor(bx[c].rawDataInGPU, (uint64_t)n, lo, hi, colbm_dev[c], stream);
+      }
+      if (e != hipSuccess) return e;
+    }
+    return rts_bitmap_and_popcount(colbm_dev.data(), ncols, (uint64_t)n, out, count, stream);
+  };
+
   rts_batch_plan *plan = nullptr;
   double batch_ms = 0;
-  if (mode == "batch" && nq > 0) {
+  std::vector<int> slot(nq, -1);  // the query's index in the batched plan
+  std::vector<rts_query_spec> plan_specs;
+  for (int q = 0; q < nq; q++)
+    if (!has_list[q]) {
+      slot[q] = (int)plan_specs.size();
+      plan_specs.push_back(specs[q]);
+    }
+  if (mode == "batch" && !plan_specs.empty()) {
     std::vector<const CODE *> dcols(ncols);
     std::vector<const BinDex *> idx(ncols);
     for (int c = 0; c < ncols; c++) {
       dcols[c] = bx[c].rawDataInGPU;
       idx[c] = &bx[c];
     }
-    if (rts_batch_plan_create(&plan, dcols.data(), idx.data(), ncols, n, nq, nullptr, stream) != hipSuccess ||
-        rts_batch_plan_set_queries(plan, specs.data(), nq, stream) != hipSuccess)
+    const int np = (int)plan_specs.size();
+    if (rts_batch_plan_create(&plan, dcols.data(), idx.data(), ncols, n, np, nullptr, stream) != hipSuccess ||
+        rts_batch_plan_set_queries(plan, plan_specs.data(), np, stream) != hipSuccess)
       return fail("batch plan");
     auto tb = std::chrono::steady_clock::now();
     if (rts_batch_plan_launch(plan, dev_out, (int64_t)stride, dev_count, stream) != hipSuccess ||
         rts_stream_synchronize(stream) != hipSuccess)
       return fail("batch launch");
     batch_ms = ms_since(tb);
-    printf("[Time] batched launch of %d queries: %.3f ms\n", nq, batch_ms);
+    printf("[Time] batched launch of %d queries: %.3f ms\n", np, batch_ms);
   }
 
   std::vector<BITS> check(words), colbm(words);
@@ -218,16 +253,26 @@ int main(int argc, char **argv) {
   for (int q = 0; q < nq; q++) {
     for (int c = 0; c < ncols; c++) {
       const rts_column_predicate *p = rts_query_file_get(qf, q, c);
-      if (p->op == BT)
-        printf("%s %u %u\n", op_name(p->op), p->t1[0], p->t2[0]);
-      else if (encode)
-        printf("%s %u [ENCODE] %u\n", op_name(p->op), p->t1[0], target(q, c, 1));
-      else
-        printf("%s %u\n", op_name(p->op), p->t1[0]);
+      std::string line = op_name(p->op), t1s, t2s;
+      for (int i = 0; i < p->ntargets; i++) {
+        t1s += (i ? "," : " ") + std::to_string(p->t1[i]);
+        if (p->op == BT) t2s += (i ? "," : " ") + std::to_string(p->t2[i]);
+      }
+      if (encode && p->op != BT) {
+        t2s = " [ENCODE]";
+        for (int i = 0; i < p->ntargets; i++) t2s += (i ? "," : " ") + std::to_string(target(q, c, 1, i));
+      }
+      printf("%s%s%s\n", line.c_str(), t1s.c_str(), t2s.c_str());
     }
     BITS *res = dev_out;
     double scan_ms = 0;
-    if (mode == "single") {
+    if (has_list[q]) {
+      res = mode == "batch" ? dev_out + (size_t)(plan_specs.size() % nout) * stride : dev_out;
+      auto ts = std::chrono::steady_clock::now();
+      if (list_query(q, res, dev_count) != hipSuccess || rts_stream_synchronize(stream) != hipSuccess)
+        return fail("target-list query");
+      scan_ms = ms_since(ts);
+    } else if (mode == "single") {
       std::vector<const BinDex *> idx(ncols);
       for (int c = 0; c < ncols; c++) idx[c] = &bx[c];
       auto ts = std::chrono::steady_clock::now();
@@ -236,24 +281,27 @@ int main(int argc, char **argv) {
         return fail("rts_bindex_query");
       scan_ms = ms_since(ts);
     } else {
-      res = dev_out + (size_t)q * stride;
+      res = dev_out + (size_t)slot[q] * stride;
     }
     auto tt = std::chrono::steady_clock::now();
     if (rts_memcpy_dtoh_async(h_result, res, words * 4, stream) != hipSuccess ||
         rts_stream_synchronize(stream) != hipSuccess)
       return fail("result transfer");
     const double xfer_ms = ms_since(tt);
-    if (mode == "single") printf("[Time] query %d: scan %.3f ms, transfer %.3f ms\n", q, scan_ms, xfer_ms);
+    if (mode == "single" || has_list[q])
+      printf("[Time] query %d: scan %.3f ms, transfer %.3f ms\n", q, scan_ms, xfer_ms);
 
     // ground truth: raw_scan of every column (bindex.cpp:1851-1892, the bt swap of
     // raw_scan_entry), ANDed across columns
     printf("[CHECK]check final result.\n");
     for (int c = 0; c < ncols; c++) {
       const rts_column_predicate *p = rts_query_file_get(qf, q, c);
-      CODE a = target(q, c, 1), b = target(q, c, 2);
-      if (p->op == BT && a > b) std::swap(a, b);
       std::fill(colbm.begin(), colbm.end(), 0u);
-      raw_scan(&bx[c], colbm.data(), a, b, (enum OPERATOR)p->op, cols[c], nullptr);
+      for (int i = 0; i < p->ntargets; i++) {  // raw_scan_entry: XOR of every target's hits
+        CODE a = target(q, c, 1, i), b = target(q, c, 2, i);
+        if (p->op == BT && a > b) std::swap(a, b);
+        raw_scan(&bx[c], colbm.data(), a, b, (enum OPERATOR)p->op, cols[c], nullptr);
+      }
       for (size_t w = 0; w < words; w++) check[w] = c ? check[w] & colbm[w] : colbm[w];
     }
     rts_compare_result r{};
@@ -276,6 +324,7 @@ int main(int argc, char **argv) {
   printf("[CHECK] %d of %d queries matched raw_scan\n", nq - mismatches, nq);
 
   if (plan) rts_batch_plan_free(plan);
+  for (BITS *b : colbm_dev) (void)rts_free(b);
   (void)rts_stream_destroy(stream);
   (void)rts_host_free(h_result);
   (void)rts_free(dev_out);

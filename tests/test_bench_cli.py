@@ -29,5 +29,6 @@ def test_multi_rank_launch_is_refused():
 def test_help_lists_the_workloads():
     p = run(["--help"])
     assert p.returncode == 0
-    for w in ("scan", "conj", "batch", "serve", "--batch-config", "--steps", "--warmup"):
+    for w in ("scan", "conj", "batch", "serve", "config1", "readprobe", "--batch-config", "--steps",
+              "--warmup"):
         assert w in p.stdout

@@ -111,3 +111,41 @@ def test_encode_mode_targets(eng, tmp_path):
     (tmp_path / "e.txt").write_text("eq 5\nexit\n")
     rc, out, err = run(["-f", path, "-b", "1", "-p", str(tmp_path / "e.txt"), "-e"])
     assert rc == 2 and "cannot be encoded" in err
+
+
+def test_target_lists_follow_the_reference_checker(eng, tmp_path):
+    """Query lines with comma-separated targets (the grammar of bindex.cpp:2168-2189): each
+    column selects the rows hit by an odd number of its targets, as the reference's checker
+    XORs every target's hits (raw_scan_entry, bindex.cpp:1894-1930); mixed with ordinary
+    queries, one query per launch and batched, against numpy."""
+    n, cols = 200_003, 2
+    data = [R.generate_column(R.DIST_UNIFORM, 4, 0, n), R.generate_column(R.DIST_ZIPF, 4, 1, n, 1.1, 2.0 ** 10)]
+    path = str(tmp_path / "data.bin")
+    np.concatenate(data).astype("<u4").tofile(path)
+    rng = np.random.default_rng(9)
+    ops = {"lt": lambda x, a, b: x < a, "le": lambda x, a, b: x <= a, "gt": lambda x, a, b: x > a,
+           "ge": lambda x, a, b: x >= a, "eq": lambda x, a, b: x == a,
+           "bt": lambda x, a, b: (x > min(a, b)) & (x < max(a, b))}
+    lines, want = [], []
+    for qi in range(10):
+        m = np.ones(n, bool)
+        for c in range(cols):
+            op = str(rng.choice(list(ops)))
+            k = 1 if qi % 3 == 0 else int(rng.integers(2, 4))
+            t1 = [int(v) for v in rng.choice(data[c], k)]
+            t2 = [int(v) for v in rng.choice(data[c], k)]
+            x = data[c].astype(np.int64)
+            col = np.zeros(n, bool)
+            for a, b in zip(t1, t2):
+                col ^= ops[op](x, a, b)
+            m &= col
+            lines.append("%s %s" % (op, ",".join(map(str, t1))) + (" " + ",".join(map(str, t2)) if op == "bt" else ""))
+        want.append(int(m.sum()))
+    lines.append("exit")
+    q = tmp_path / "q.txt"
+    q.write_text("\n".join(lines) + "\n")
+    for mode in ("single", "batch"):
+        rc, out, err = run(["-f", path, "-b", str(cols), "-p", str(q), "-m", mode])
+        assert rc == 0, (rc, err[-2000:], out[-3000:])
+        h = hits(out)
+        assert [b for _, b in h] == want and all(a == b for a, b in h), (mode, h, want)
