@@ -395,6 +395,7 @@ def bench_scan(args, R):
             "bindex_query_ms_per_step": ms_b / steps,
             "bindex_query_rows_per_s": n * steps / (ms_b / 1e3),
             "bindex_query_roofline": {"bound": "hbm", "alg_bytes": b_alg,
+                                      "traffic": load_traffic("bindex_query_1e8_kernel_stats.txt"),
                                       "achieved_GBps": b_alg / (ms_b / 1e3 / steps) / 1e9,
                                       "frac": b_alg / (ms_b / 1e3 / steps) / 1e9 / HBM_PEAK_GBS},
             "bindex_query_512bins_ms_per_step": ms_b512 / steps,
@@ -754,10 +755,12 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
         chosen, tune_ms = plan.autotune(out, stride, cands, reps=2, cache_path=tune_cache, stream=stream)
         stages = [len(cands)]
     else:
-        shapes = [R.batch_config(t, g, 4, 1, nt) for t in (1024, 2048) for g in (32, 64) for nt in (256, 512)]
+        # launch shape x bin count jointly, then the planner's direct weight on the winner
+        shapes = [R.batch_config(t, g, 4, 1, nt, bins=b) for b in (128, 256, 512) for t in (1024, 2048)
+                  for g in (32, 64) for nt in (256, 512)]
         shape, _ = plan.autotune(out, stride, shapes, reps=2, stream=stream)
-        s = shape.astuple()[:5]
-        plans = [R.batch_config(*s, bins=b, direct_weight=w) for b in (128, 256, 512) for w in (0, 40, 80)]
+        s = shape.astuple()[:6]
+        plans = [R.batch_config(*s, direct_weight=w) for w in (0, 40, 80)]
         chosen, tune_ms = plan.autotune(out, stride, plans, reps=2, cache_path=tune_cache, stream=stream)
         stages = [len(shapes), len(plans)]
     tune_s = time.perf_counter() - t0
@@ -841,7 +844,7 @@ def bench_batch(R, table, nq=4096, seed=44, steps=5, tune_cache=None, config=Non
         "roofline": {"bound": "hbm", "alg_bytes": floor_bytes,
                      "achieved_GBps": floor_bytes / (ms / steps / 1e3) / 1e9,
                      "frac": floor_bytes / (ms / steps / 1e3) / 1e9 / HBM_PEAK_GBS,
-                     "traffic": load_traffic("batch_4096q_1e8x4_traffic.txt")},
+                     "traffic": load_traffic("batch_4096q_1e8x4_kernel_stats.txt")},
         "cpu_baseline": cpu,
         "autotune": {"chosen": list(chosen.astuple()), "best_ms": tune_ms, "search_s": tune_s,
                      "candidates": stages},
@@ -874,9 +877,10 @@ def serve_queries(rng, table_cols, nq, ncols_max):
 
 
 def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000_000_000,
-                seed=55, pool_gb=32, batch_config=None):
+                seed=55, pool_gb=32, batch_config=None, bins=256):
     """Config 5: two resident tables, T1 = 2^26 rows x 2 uniform columns, T2 = n2 rows x 4
-    columns (uniform, normal, Zipf 1.1, Zipf 1.5), a BinDex per column; Poisson arrivals at
+    columns (uniform, normal, Zipf 1.1, Zipf 1.5), a BinDex of `bins` bins per column (the
+    count the autotuner picks for the config-4 batch); Poisson arrivals at
     each rate, every query on T1 or T2 with probability 1/2, shapes as config 4; admission by
     the result pool's free HBM; continuous batching over 3 streams."""
     t_setup = time.time()
@@ -885,7 +889,7 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
 
     t1_cols = [R.generate_column(0, seed, c, n1) for c in range(2)]
     t1_d = [R.DeviceBuffer.from_numpy(c) for c in t1_cols]
-    t1_ix = [R.BinDexIndex(d, n1) for d in t1_d]
+    t1_ix = [R.BinDexIndex(d, n1, bins=bins) for d in t1_d]
     t2_d, t2_ix, t2_samples, t2_cols = [], [], [], []
     rng = np.random.default_rng(seed)
     for c, (k, p0, p1) in enumerate(TABLE_KINDS):
@@ -893,7 +897,7 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
         t2_samples.append(np.sort(col[rng.integers(0, n2, 1_000_000)]))
         t2_d.append(R.DeviceBuffer.from_numpy(col))
         t2_cols.append(col)  # kept on the host for the CPU baseline and the oracle checks
-        t2_ix.append(R.BinDexIndex(t2_d[-1], n2))
+        t2_ix.append(R.BinDexIndex(t2_d[-1], n2, bins=bins))
     t1_samples = [np.sort(c[rng.integers(0, n1, 1_000_000)]) for c in t1_cols]
     setup_s = time.time() - t_setup
     cfg = R.server_config(max_batch=256, streams=3, pool_bytes=pool_gb << 30)
@@ -904,7 +908,7 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
     tid2 = srv.add_table(t2_d, t2_ix, n2)
     srv.start()
     progress("served tables ready (%.1f s)" % setup_s)
-    out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb,
+    out = {"setup_s": round(setup_s, 1), "rows": [n1, n2], "pool_gb": pool_gb, "bins": bins,
            "batch_config": list(cfg.batch.astuple()), "rates": []}
     for rate, secs in zip(rates, seconds):
         nq = int(rate * secs)
@@ -1005,7 +1009,7 @@ def main():
     ap.add_argument("--serve-rows", type=int, default=1_000_000_000,
                     help="rows of the large serving table")
     ap.add_argument("--batch-config", default=None,
-                    help="tile_words,queries_per_group,blocks_per_cu[,order[,threads]] for the "
+                    help="tile_words,queries_per_group,blocks_per_cu[,order[,threads[,bins[,direct_weight]]]] for the "
                          "batch workload (skips the autotuner) or the serve workload's batches")
     ap.add_argument("--no-extra", action="store_true",
                     help="skip the conjunction, batched and serving legs of the default run")
@@ -1070,7 +1074,12 @@ def main():
                 res["extra"]["aux_error"] = "%s: %s" % (type(e).__name__, e)
             gc.collect()
             try:
-                res["extra"]["serve"] = bench_serve(R, n2=args.serve_rows)
+                # the request loop batches with the launch shape tuned for the config-4 batch
+                chosen = res["extra"].get("batch", {}).get("autotune", {}).get("chosen")
+                cfg = tuple(chosen[:5]) + (0, chosen[6]) if chosen else None
+                # (at most 256 bins: 512 on the 1e9 x 4 table would be 255 GB of filter vectors)
+                bins = min(256, chosen[5] or 128) if chosen else 128
+                res["extra"]["serve"] = bench_serve(R, n2=args.serve_rows, batch_config=cfg, bins=bins)
             except Exception as e:  # noqa: BLE001
                 res["extra"]["serve_error"] = "%s: %s" % (type(e).__name__, e)
     print(json.dumps(res), flush=True)

@@ -123,8 +123,10 @@ void rts_bindex_free(BinDex *bindex);
  * bindex_cuda.cpp:1860-1890): write a built index to a binary snapshot file, and load one
  * into device memory without sorting or rebuilding the filter vectors.  `dev_column` is the
  * indexed column already on the device (kept as rawDataInGPU for the refine surface; may
- * be NULL).  A snapshot whose header, host arrays, refine-list rows or per-array digests
- * do not check out is refused with hipErrorInvalidValue before the index can be used.
+ * be NULL; when given it must hold the snapshot's n rows).  A snapshot whose header, host
+ * arrays, refine-list rows or per-array digests do not check out, or whose indexed column
+ * differs from `dev_column`, is refused with hipErrorInvalidValue before the index can be
+ * used.
  * Synchronous. */
 hipError_t rts_bindex_save(const BinDex *bindex, const char *path);
 hipError_t rts_bindex_load(BinDex *bindex, const char *path, const CODE *dev_column);

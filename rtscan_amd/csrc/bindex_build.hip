@@ -435,7 +435,7 @@ void bindex_free(BinDex *bx) {
 // File layout (host byte order, little-endian on MI355X hosts):
 //   "RTSBDX" 0 2 | u32 version, u32 bins, u32 sub, u32 pad | i64 n, words, fv_stride, ntiles,
 //   nb, nblocks | u32 data_max, u32 pad | u64 digests: host arrays, sorted_val, sorted_pos,
-//   fvs, bent |
+//   fvs, bent, the indexed column (0 when the index had none) |
 //   starts[128] u32 | area_start[128] i64 | area_len[128] i64 | estarts[bins] u32 |
 //   boff[bins + 1] u32 | block_first[nblocks] u32 | block_last[nblocks] u32 |
 //   sorted_val[n] u32 | sorted_pos[n] i32 | fvs[(bins - 1) * fv_stride] u32 | bent[nb] (u32, u32)
@@ -462,6 +462,7 @@ struct SnapHeader {
   int64_t n, words, fv_stride, ntiles, nb, nblocks;
   uint32_t data_max, pad1;
   uint64_t dg_host, dg_val, dg_pos, dg_fvs, dg_bent;
+  uint64_t dg_col;  // digest of the indexed column at save time (0: not known)
 };
 
 bool put(FILE *f, const void *p, size_t bytes) { return fwrite(p, 1, bytes, f) == bytes; }
@@ -573,6 +574,7 @@ hipError_t bindex_save(const BinDex *bx, const char *path) {
   if (e == hipSuccess) e = device_digest(d->sorted_pos, d->n, &h.dg_pos);
   if (e == hipSuccess) e = device_digest(d->fvs, (int64_t)fv_words, &h.dg_fvs);
   if (e == hipSuccess) e = device_digest(d->bent, 2 * d->nb, &h.dg_bent);
+  if (e == hipSuccess && bx->rawDataInGPU) e = device_digest(bx->rawDataInGPU, d->n, &h.dg_col);
   if (e != hipSuccess) return e;
   FILE *f = fopen(path, "wb");
   if (!f) return hipErrorInvalidValue;
@@ -665,6 +667,11 @@ hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
   if (e == hipSuccess && dg != h.dg_fvs) e = hipErrorInvalidValue;
   if (e == hipSuccess) e = device_digest(d->bent, 2 * h.nb, &dg);
   if (e == hipSuccess && dg != h.dg_bent) e = hipErrorInvalidValue;
+  // the column given with the snapshot must be the one that was indexed
+  if (e == hipSuccess && col && h.dg_col) {
+    e = device_digest(col, h.n, &dg);
+    if (e == hipSuccess && dg != h.dg_col) e = hipErrorInvalidValue;
+  }
   if (e == hipSuccess)
     e = hipMemcpy(d->boff, d->boff_host.data(), (d->bins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_tile_directory(d, d->ntiles, kTileRows, d->bdir, ref_stream());

@@ -5,7 +5,7 @@ import struct
 import numpy as np
 
 AREAS = 128
-HEADER = struct.Struct("<8sIIII6qII5Q")
+HEADER = struct.Struct("<8sIIII6qII6Q")
 
 
 def fnv1a(chunks):
@@ -19,7 +19,7 @@ def fnv1a(chunks):
 def parse(raw):
     """Header fields and the host arrays of a snapshot, plus the payload offset."""
     f = dict(zip(("magic", "version", "bins", "sub", "pad0", "n", "words", "fv_stride", "ntiles", "nb",
-                  "nblocks", "data_max", "pad1", "dg_host", "dg_val", "dg_pos", "dg_fvs", "dg_bent"),
+                  "nblocks", "data_max", "pad1", "dg_host", "dg_val", "dg_pos", "dg_fvs", "dg_bent", "dg_col"),
                  HEADER.unpack_from(raw, 0)))
     off = HEADER.size
     arrays = {}
@@ -40,7 +40,7 @@ def build(f, arrays, payload=b""):
     f["dg_host"] = fnv1a([arrays[k].tobytes() for k in order])
     head = HEADER.pack(*[f[k] for k in ("magic", "version", "bins", "sub", "pad0", "n", "words", "fv_stride",
                                        "ntiles", "nb", "nblocks", "data_max", "pad1", "dg_host", "dg_val",
-                                       "dg_pos", "dg_fvs", "dg_bent")])
+                                       "dg_pos", "dg_fvs", "dg_bent", "dg_col")])
     return head + b"".join(arrays[k].tobytes() for k in order) + payload
 
 
@@ -59,7 +59,7 @@ def synthetic(n=128 * 4000, sub=1):
     f = {"magic": b"RTSBDX\x00\x02", "version": 2, "bins": bins, "sub": sub, "pad0": 0, "n": n,
          "words": words, "fv_stride": (words + 63) // 64 * 64, "ntiles": (n + 32767) // 32768, "nb": 0,
          "nblocks": AREAS * nblk, "data_max": int(starts[-1]) + per * 16, "pad1": 0, "dg_host": 0,
-         "dg_val": 0, "dg_pos": 0, "dg_fvs": 0, "dg_bent": 0}
+         "dg_val": 0, "dg_pos": 0, "dg_fvs": 0, "dg_bent": 0, "dg_col": 0}
     arrays = {"starts": starts, "area_start": area_start, "area_len": area_len, "estarts": estarts,
               "boff": np.zeros(bins + 1, np.uint32), "block_first": block_first,
               "block_last": block_first.copy()}

@@ -76,3 +76,17 @@ def test_damaged_snapshot_is_refused(eng, tmp_path):
     short.write_bytes(bytes(raw[: len(raw) // 2]))  # truncated
     with pytest.raises(R.HipError):
         R.BinDexIndex(d, n, snapshot=str(short))
+
+
+def test_snapshot_refuses_another_column_of_the_same_length(eng, tmp_path):
+    n = 200_003
+    x = R.generate_column(R.DIST_UNIFORM, 8, 0, n)
+    y = x.copy()
+    y[n // 2] ^= 1  # one row differs
+    dx, dy = R.DeviceBuffer.from_numpy(x), R.DeviceBuffer.from_numpy(y)
+    ix = R.BinDexIndex(dx, n)
+    path = str(tmp_path / "c.rtsbdx")
+    ix.save(path)
+    R.BinDexIndex(dx, n, snapshot=path).free()  # its own column: accepted
+    with pytest.raises(R.HipError):
+        R.BinDexIndex(dy, n, snapshot=path)

@@ -123,3 +123,36 @@ def test_replay_reports_every_request(eng, tables):
     s = srv.stats()
     assert s["completed"] == nq and s["pool_in_use"] == 0
     srv.destroy()
+
+
+class _Offset:
+    def __init__(self, buf, off):
+        self.ptr = buf.ptr + off
+
+
+def test_failed_batches_end_the_replay_and_later_replays_run(eng, tables):
+    """A batch that fails (forced from the host: a table whose column is not 16-byte aligned
+    and has no index, so planning its direct form is refused) fails its requests: the replay
+    returns an error once every one of them is counted, none is left queued or holding pool
+    memory, and a later replay on a good table is not cut short by that earlier error."""
+    (c1, d1, i1), _ = tables
+    n = c1[0].size - 1
+    srv = eng.Server(eng.server_config(max_batch=8, streams=2, pool_bytes=64 << 20))
+    bad = srv.add_table([_Offset(d1[0], 4)], [None], n)
+    good = srv.add_table(d1, i1, c1[0].size)
+    srv.start()
+    rng = np.random.default_rng(12)
+    qs = [[(0, 5, 1 << 31, 0)] for _ in range(40)]
+    arrivals = np.cumsum(rng.exponential(2e5, len(qs))).astype(np.uint64)
+    with pytest.raises(eng.HipError):
+        srv.replay([bad] * len(qs), qs, arrivals)
+    st = srv.stats()
+    assert st["submitted"] == st["completed"] == len(qs)
+    assert st["pool_in_use"] == 0
+    qs2 = [random_query(rng, c1) for _ in range(60)]
+    lat, cnt, _ = srv.replay([good] * len(qs2), qs2, np.cumsum(rng.exponential(1e5, len(qs2))).astype(np.uint64))
+    for q, c in zip(qs2, cnt):
+        assert int(c) == int(truth(c1, q).sum())
+    st = srv.stats()
+    assert st["submitted"] == st["completed"] == len(qs) + len(qs2) and st["pool_in_use"] == 0
+    srv.destroy()
