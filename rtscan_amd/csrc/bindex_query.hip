@@ -54,49 +54,58 @@ __device__ __forceinline__ uint4 base_words(const BoundDev &b, int64_t w, int64_
 
 constexpr int kListU = 2;  // list entries per thread per list per round
 
-template <int C, int MODE, bool COUNT>
+// A workgroup owns M directory tiles: TQ = M * kTileWords output words, kPer = TQ / kBlock
+// words per thread (4 or 8, one or two uint4 per filter vector).
+template <int C, int MODE, bool COUNT, int M>
 __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t n, int64_t words,
-                                                         int64_t ntiles, BITS *out, bool out_vec,
+                                                         int64_t ndir, BITS *out, bool out_vec,
                                                          CountSink sink) {
   // One toggle array per column for both bounds: with L(v) = fv ^ t_v the rows below v,
   // L(lo) is inside L(hi + 1), so L(hi + 1) & ~L(lo) = (fv_hi ^ fv_lo) ^ (t_lo ^ t_hi); the two
   // bounds' lists cover different bins (or, both bounds in one bin, a single list) and
   // toggle disjoint rows of the same array.
-  __shared__ alignas(16) uint32_t tog[C][kTileWords];
-  constexpr int kPer = kTileWords / kBlock;  // 4 words per thread
-  static_assert(kPer == 4, "one uint4 of output words per thread");
+  constexpr int TQ = M * kTileWords;
+  constexpr int kPer = TQ / kBlock;  // 4 or 8 words per thread
+  constexpr int kVec = kPer / 4;     // uint4 per filter vector per thread
+  static_assert(kPer % 4 == 0, "whole uint4 of output words per thread");
+  __shared__ alignas(16) uint32_t tog[C][TQ];
   const uint32_t last_mask = tail_mask((uint64_t)n);
+  const int64_t ntiles = (words + TQ - 1) / TQ;
   uint32_t pc = 0;
   for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     // The filter-vector words and the lists' directory words do not depend on the refine:
     // issue them first, so that one round trip covers both and the second round trip is
     // the refine entries.
-    const int lw = threadIdx.x * kPer;
-    const int64_t w = t * kTileWords + lw;
-    uint4 lo_w[C], hi_w[C];
+    const int lw = threadIdx.x * 4;  // the thread's uint4 within each 1024-word part of the tile
+    const int64_t d0 = t * M, d1 = d0 + M < ndir ? d0 + M : ndir;  // directory tiles covered
+    uint4 lo_w[C][kVec], hi_w[C][kVec];
     uint32_t lo_e[C][2], hi_e[C][2];
 #pragma unroll
     for (int c = 0; c < C; c++) {
-      if (w < words) {
-        lo_w[c] = base_words(q.lo[c], w, words);
-        hi_w[c] = base_words(q.hi[c], w, words);
+#pragma unroll
+      for (int k = 0; k < kVec; k++) {
+        const int64_t w = t * TQ + k * kTileWords + lw;
+        if (w < words) {
+          lo_w[c][k] = base_words(q.lo[c], w, words);
+          hi_w[c][k] = base_words(q.hi[c], w, words);
+        }
       }
       lo_e[c][0] = lo_e[c][1] = hi_e[c][0] = hi_e[c][1] = 0;
       if (q.lo[c].dir) {
-        lo_e[c][0] = q.lo[c].dir[t];
-        lo_e[c][1] = q.lo[c].dir[t + 1];
+        lo_e[c][0] = q.lo[c].dir[d0];
+        lo_e[c][1] = q.lo[c].dir[d1];
       }
       if (q.hi[c].dir) {
-        hi_e[c][0] = q.hi[c].dir[t];
-        hi_e[c][1] = q.hi[c].dir[t + 1];
+        hi_e[c][0] = q.hi[c].dir[d0];
+        hi_e[c][1] = q.hi[c].dir[d1];
       }
     }
-    for (int i = threadIdx.x; i < C * kTileWords; i += kBlock) (&tog[0][0])[i] = 0;
+    for (int i = threadIdx.x; i < C * TQ; i += kBlock) (&tog[0][0])[i] = 0;
     __syncthreads();
     // The 2C lists' tile slices: every list's next kListU entries per thread are loaded
     // before any of them is applied, so one round trip covers all lists (a tile's slice of
-    // one list is about kTileRows / bins entries: one or two rounds in practice).
-    const uint32_t row0 = (uint32_t)(t * kTileRows);
+    // one list is about M * kTileRows / bins entries: one or two rounds in practice).
+    const uint32_t row0 = (uint32_t)(t * TQ * 32);
     for (uint32_t off = 0;; off += kListU * kBlock) {
       uint2 ent[2 * C][kListU];
       bool more = false;
@@ -118,6 +127,7 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
 #pragma unroll
         for (int u = 0; u < kListU; u++) {
           const uint2 pv = ent[l][u];
+          // rows of the slice lie in [row0, row0 + TQ * 32): the word index is < TQ
           if (pv.x != 0xFFFFFFFFu && pv.y - b.thr <= b.span)
             atomicXor(&tog[l / 2][(pv.x - row0) >> 5], 0x80000000u >> (pv.x & 31));
         }
@@ -125,12 +135,16 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
       if (!more) break;
     }
     __syncthreads();
-    if (w < words) {
+#pragma unroll
+    for (int k = 0; k < kVec; k++) {
+      const int lwk = k * kTileWords + lw;  // this uint4's first word in the tile
+      const int64_t w = t * TQ + lwk;
+      if (w >= words) continue;
       uint4 acc = make_uint4(~0u, ~0u, ~0u, ~0u);
 #pragma unroll
       for (int c = 0; c < C; c++) {
-        const uint4 lo = lo_w[c], hi = hi_w[c];
-        const uint4 tg = *reinterpret_cast<const uint4 *>(&tog[c][lw]);
+        const uint4 lo = lo_w[c][k], hi = hi_w[c][k];
+        const uint4 tg = *reinterpret_cast<const uint4 *>(&tog[c][lwk]);
         acc.x &= hi.x ^ lo.x ^ tg.x;
         acc.y &= hi.y ^ lo.y ^ tg.y;
         acc.z &= hi.z ^ lo.z ^ tg.z;
@@ -180,24 +194,39 @@ static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   return b;
 }
 
-template <int C, int MODE>
-static hipError_t launch_query(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
-                               unsigned long long *count, hipStream_t s) {
+// Two directory tiles (2048 words, two uint4 per filter vector per thread) per workgroup on
+// columns of at least 2^24 rows, for one or two columns: half the workgroups, each with
+// twice the loads in flight (1e8 rows: 14.5 -> 13.8 us at 128 bins, 13.9 -> 12.4 us at 512).
+// With three or four columns the doubled registers cost occupancy and it measured slower
+// (28.3 -> 30.3 us and 33.5 -> 37.3 us), so those keep one tile per workgroup.
+constexpr int64_t kWideQueryRows = (int64_t)1 << 24;
+
+template <int C, int MODE, int M>
+static hipError_t launch_query_m(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
+                                 unsigned long long *count, hipStream_t s) {
   const bool vec = ((uintptr_t)out & 15) == 0;
-  const unsigned grid = (unsigned)std::min<int64_t>(d->ntiles, (int64_t)device_cu_count() * 32);
+  const int64_t ntiles = (d->words + M * kTileWords - 1) / (M * kTileWords);
+  const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)device_cu_count() * 32);
   if (count) {
     CountSink sink{count, stream_count_partials(s)};
     if (!sink.partials) return hipErrorOutOfMemory;
-    hipLaunchKernelGGL((k_bindex_query<C, MODE, true>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
+    hipLaunchKernelGGL((k_bindex_query<C, MODE, true, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
                        d->words, d->ntiles, out, vec, sink);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_count_finalize(sink, grid, s);
   } else {
-    hipLaunchKernelGGL((k_bindex_query<C, MODE, false>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
+    hipLaunchKernelGGL((k_bindex_query<C, MODE, false, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
                        d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr});
   }
   return hipGetLastError();
+}
+
+template <int C, int MODE>
+static hipError_t launch_query(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
+                               unsigned long long *count, hipStream_t s) {
+  if (C <= 2 && d->n >= kWideQueryRows) return launch_query_m<C, MODE, 2>(q, d, out, count, s);
+  return launch_query_m<C, MODE, 1>(q, d, out, count, s);
 }
 
 template <int C>

@@ -283,3 +283,35 @@ def test_reference_shaped_caller_runs_on_the_engine(eng):
     r = subprocess.run([exe, "gpu", "1000003"], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stderr[-2000:]
     assert r.stdout.startswith("ok ")
+
+
+@pytest.mark.parametrize("n", [(1 << 24) + 12345, (1 << 24) + 32 * 1024 * 3 + 7])
+def test_wide_workgroup_queries_on_large_columns(eng, n):
+    """Columns of at least 2^24 rows run the query kernel with 2048-word workgroups (two
+    directory tiles each); ragged lengths leave the last workgroup a partial tile or a lone
+    directory tile. Single- and multi-column queries equal numpy, at 128 and 512 bins."""
+    rng = np.random.default_rng(n % 1000)
+    xs = [eng.generate_column(eng.DIST_UNIFORM, 9, 0, n),
+          eng.generate_column(eng.DIST_ZIPF, 9, 2, n, 1.1, 2.0 ** 20)]
+    ds = [upload(eng, x) for x in xs]
+    W = (n + 31) // 32
+    out = eng.DeviceBuffer(4 * W)
+    cnt = eng.DeviceBuffer(8)
+    for bins in (128, 512):
+        ixs = [eng.BinDexIndex(d, n, bins=bins) for d in ds]
+        for _ in range(6):
+            cs = [0] if rng.random() < 0.5 else [0, 1]
+            rs = []
+            for c in cs:
+                s = np.sort(xs[c][rng.integers(0, n, 10000)])
+                a, b = sorted(int(v) for v in rng.integers(0, s.size, 2))
+                rs.append((int(s[a]), int(s[b])))
+            out.fill(0x5C)
+            eng.bindex_query([ixs[c] for c in cs], [r[0] for r in rs], [r[1] for r in rs], out, cnt)
+            m = np.ones(n, bool)
+            for c, (lo, hi) in zip(cs, rs):
+                m &= interval_mask(xs[c], lo, hi)
+            assert np.array_equal(out.to_numpy(np.uint32, W), pack_rows(m)), (bins, cs, rs)
+            assert int(cnt.to_numpy(np.uint64, 1)[0]) == int(m.sum())
+        for ix in ixs:
+            ix.free()
