@@ -198,7 +198,9 @@ static BoundDev make_bound(const rts_bindex_dev *d, const BoundSpec &s) {
 // columns of at least 2^24 rows, for one or two columns: half the workgroups, each with
 // twice the loads in flight (1e8 rows: 14.5 -> 13.8 us at 128 bins, 13.9 -> 12.4 us at 512).
 // With three or four columns the doubled registers cost occupancy and it measured slower
-// (28.3 -> 30.3 us and 33.5 -> 37.3 us), so those keep one tile per workgroup.
+// (28.3 -> 30.3 us and 33.5 -> 37.3 us), so those keep one tile per workgroup. Four tiles
+// for one column measured no faster than two (12.9 vs 12.6 us at 128 bins, 11.4 vs 11.2 at
+// 512).
 constexpr int64_t kWideQueryRows = (int64_t)1 << 24;
 
 template <int C, int MODE, int M>
@@ -208,8 +210,8 @@ static hipError_t launch_query_m(const QueryArgs<C> &q, const rts_bindex_dev *d,
   const int64_t ntiles = (d->words + M * kTileWords - 1) / (M * kTileWords);
   const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)device_cu_count() * 32);
   if (count) {
-    CountSink sink{count, stream_count_partials(s)};
-    if (!sink.partials) return hipErrorOutOfMemory;
+    const CountSink sink = stream_count_sink(count, s);
+    if (!sink.acc) return hipErrorOutOfMemory;
     hipLaunchKernelGGL((k_bindex_query<C, MODE, true, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
                        d->words, d->ntiles, out, vec, sink);
     hipError_t e = hipGetLastError();
@@ -217,7 +219,7 @@ static hipError_t launch_query_m(const QueryArgs<C> &q, const rts_bindex_dev *d,
     return launch_count_finalize(sink, grid, s);
   } else {
     hipLaunchKernelGGL((k_bindex_query<C, MODE, false, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
-                       d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr});
+                       d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr, nullptr});
   }
   return hipGetLastError();
 }

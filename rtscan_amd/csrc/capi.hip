@@ -95,10 +95,11 @@ thread_local ThreadScratch t_scratch;
 }  // namespace
 
 hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t host_bytes, void **dptr,
-                          void **hptr) {
+                          void **hptr, bool *created) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
+  if (created) *created = false;
   std::vector<ScratchEntry> &tab = t_scratch.entries;
   for (const ScratchEntry &x : tab) {
     if (x.kind == kind && x.dev == dev && x.stream == (uintptr_t)stream) {
@@ -107,6 +108,7 @@ hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t
       return hipSuccess;
     }
   }
+  if (created) *created = true;
   ScratchEntry x{kind, dev, (uintptr_t)stream, nullptr, nullptr};
   if (dev_bytes) e = hipMalloc(&x.dptr, dev_bytes);
   if (e == hipSuccess && host_bytes) e = hipHostMalloc(&x.hptr, host_bytes, 0);
@@ -125,6 +127,16 @@ hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t
 }
 
 size_t thread_scratch_entries() { return t_scratch.entries.size(); }
+
+CountSink stream_count_sink(unsigned long long *out, hipStream_t stream) {
+  void *p = nullptr;
+  bool created = false;
+  if (thread_scratch(kScratchCountAcc, stream, kCountAccBytes, 0, &p, nullptr, &created) != hipSuccess)
+    return CountSink{out, nullptr, nullptr};
+  // the counting kernels leave the accumulators at zero; a new set starts there too
+  if (created && hipMemsetAsync(p, 0, kCountAccBytes, stream) != hipSuccess) return CountSink{out, nullptr, nullptr};
+  return CountSink{out, nullptr, static_cast<unsigned long long *>(p)};
+}
 
 uint32_t *stream_count_partials(hipStream_t stream) {
   void *p = nullptr;
@@ -163,7 +175,7 @@ __global__ __launch_bounds__(kFinalizeThreads) void k_count_finalize(const uint3
 }
 
 hipError_t launch_count_finalize(const CountSink &sink, unsigned nblocks, hipStream_t stream) {
-  if (!sink.out) return hipSuccess;
+  if (!sink.out || sink.acc) return hipSuccess;
   hipLaunchKernelGGL(k_count_finalize, dim3(1), dim3(kFinalizeThreads), 0, stream, sink.partials,
                      nblocks, sink.out);
   return hipGetLastError();

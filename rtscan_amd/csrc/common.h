@@ -70,19 +70,29 @@ __device__ __forceinline__ uint32_t store_word(BITS *out, uint64_t w, uint32_t v
   return v;
 }
 
-// Popcounts: every block of a counting kernel stores its partial count in its own slot of
-// a per-stream scratch array (plain stores, no same-address atomics: thousands of blocks
-// serialising on one counter cost more than the scan itself), then a one-block kernel on
-// the same stream sums the slots into the caller's output.
+// Popcounts of a launch, two ways.  Accumulators (the default): every workgroup adds
+// (1 << 44) | its count to its group's 64-bit accumulator in one atomic (group =
+// blockIdx.x % kCountGroups, the accumulators 128 bytes apart so the groups' atomics go to
+// different places); the workgroup whose ticket completes its group adds the group's total
+// to the final accumulator the same way, and the one completing that writes the count and
+// leaves every accumulator at zero for the next launch.  Every exchange is one atomic, so
+// no fence is needed and no second kernel runs.  Partials (the allocation-free entry
+// point, whose fixed scratch cannot be trusted to start at zero): every workgroup stores
+// its count in its own slot and a one-workgroup kernel sums the slots.
 constexpr int kMaxCountBlocks = 1 << 16;
+constexpr int kCountGroups = 64;
+constexpr int kCountAccStride = 16;  // 128 bytes between accumulators
+constexpr size_t kCountAccBytes = (kCountGroups + 1) * kCountAccStride * sizeof(unsigned long long);
+constexpr int kCountTicketShift = 44;  // counts below 2^44, at most 2^20 workgroups per group
 
 struct CountSink {
   unsigned long long *out;  // nullptr: no count wanted
-  uint32_t *partials;       // kMaxCountBlocks slots
+  uint32_t *partials;       // kMaxCountBlocks slots, or nullptr
+  unsigned long long *acc;  // the accumulators (zero between launches), or nullptr
 };
 
-// Block-wide sum of the threads' counts into partials[blockIdx.x].  Every block of the
-// grid must call it exactly once; the grid must not exceed kMaxCountBlocks blocks.
+// Block-wide sum of the threads' counts into the launch's count.  Every block of the grid
+// must call it exactly once; the grid must not exceed kMaxCountBlocks blocks.
 __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink) {
   __shared__ uint32_t part[kBlock / kWave];
   pc = wave_sum(pc);
@@ -92,7 +102,26 @@ __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink)
   if (threadIdx.x == 0) {
     uint32_t s = 0;
     for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += part[i];
-    sink.partials[blockIdx.x] = s;
+    if (!sink.acc) {
+      sink.partials[blockIdx.x] = s;
+      return;
+    }
+    constexpr unsigned long long kLow = (1ull << kCountTicketShift) - 1;
+    const unsigned g = blockIdx.x % kCountGroups;
+    const unsigned in_group = (gridDim.x - g - 1) / kCountGroups + 1;  // blocks of group g
+    unsigned long long *ga = sink.acc + (size_t)g * kCountAccStride;
+    const unsigned long long old = atomicAdd(ga, (1ull << kCountTicketShift) | s);
+    if ((old >> kCountTicketShift) + 1 != in_group) return;
+    // this block completes group g: every block of the group has added, so the group's
+    // accumulator can be cleared for the next launch
+    const unsigned long long gsum = (old & kLow) + s;
+    atomicExch(ga, 0ull);
+    const unsigned groups = gridDim.x < (unsigned)kCountGroups ? gridDim.x : (unsigned)kCountGroups;
+    unsigned long long *fa = sink.acc + (size_t)kCountGroups * kCountAccStride;
+    const unsigned long long old2 = atomicAdd(fa, (1ull << kCountTicketShift) | gsum);
+    if ((old2 >> kCountTicketShift) + 1 != groups) return;
+    *sink.out = (old2 & kLow) + gsum;
+    atomicExch(fa, 0ull);
   }
 }
 
@@ -106,14 +135,19 @@ inline hipStream_t ref_stream() { return hipStreamPerThread; }
 // Scratch of (current device, stream, calling host thread), allocated on first use and
 // kept by the calling thread (a bounded number per thread, freed when the thread exits).
 // Not for use under graph capture: the first use on a stream allocates.
-enum ScratchKind : int { kScratchCountPartials = 0, kScratchCheck = 1 };
+enum ScratchKind : int { kScratchCountPartials = 0, kScratchCheck = 1, kScratchCountAcc = 2 };
 hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t host_bytes, void **dptr,
-                          void **hptr);
+                          void **hptr, bool *created = nullptr);
 size_t thread_scratch_entries();
 // Count partials (kMaxCountBlocks slots) of (device, stream, calling thread).
 uint32_t *stream_count_partials(hipStream_t stream);
+// The count sink of a counted launch on `stream` writing *out: the calling thread's
+// accumulators for the stream (zeroed when first made, on the stream), or sink.acc ==
+// nullptr when they cannot be had (the launch then fails with hipErrorOutOfMemory).
+CountSink stream_count_sink(unsigned long long *out, hipStream_t stream);
 
-// Sum partials[0, nblocks) into *out on `stream` (one small launch).
+// For a partials sink: sum partials[0, nblocks) into *out on `stream` (one small launch).
+// For an accumulator sink: nothing (the counting kernel wrote the count).
 hipError_t launch_count_finalize(const CountSink &sink, unsigned nblocks, hipStream_t stream);
 
 // Number of compute units of the current device (cached per device).

@@ -268,11 +268,11 @@ template <int C, int MODE>
 static hipError_t launch_scan(const ColPreds<C> &p, uint64_t n, BITS *out,
                               unsigned long long *count, hipStream_t s, uint32_t *partials) {
   if (count) {
-    CountSink sink{count, partials ? partials : stream_count_partials(s)};
-    if (!sink.partials) return hipErrorOutOfMemory;
+    const CountSink sink = partials ? CountSink{count, partials, nullptr} : stream_count_sink(count, s);
+    if (!sink.partials && !sink.acc) return hipErrorOutOfMemory;
     return launch_scan_c<C, MODE, true>(p, n, out, sink, s);
   }
-  return launch_scan_c<C, MODE, false>(p, n, out, CountSink{nullptr, nullptr}, s);
+  return launch_scan_c<C, MODE, false>(p, n, out, CountSink{nullptr, nullptr, nullptr}, s);
 }
 
 // Empty predicate: all-zero bitmap (set/and modes), zero count.
@@ -345,8 +345,8 @@ hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS 
     hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, s, count);
     return hipGetLastError();
   }
-  uint32_t *partials = count ? stream_count_partials(s) : nullptr;
-  if (count && !partials) return hipErrorOutOfMemory;
+  const CountSink counted = count ? stream_count_sink(count, s) : CountSink{nullptr, nullptr, nullptr};
+  if (count && !counted.acc) return hipErrorOutOfMemory;
   // Chains of more than kMaxAnd inputs: AND the first kMaxAnd into `out`, continue from it.
   int done = 0;
   while (true) {
@@ -362,7 +362,7 @@ hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS 
     for (int i = 0; i < k; i++) vec = vec && aligned16(set.bm[i]);
     const unsigned grid = grid_for(vec ? (words + 3) / 4 : words, kBlock, stream_grid_cap());
     const uint32_t lm = tail_mask(n);
-    const CountSink sink{last ? count : nullptr, last ? partials : nullptr};
+    const CountSink sink = last ? counted : CountSink{nullptr, nullptr, nullptr};
 #define RTS_LAUNCH_AND(V, C)                                                                    \
   hipLaunchKernelGGL((k_and_popcount<V, C>), dim3(grid), dim3(kBlock), 0, s, set, k, words, lm, \
                      out, sink)

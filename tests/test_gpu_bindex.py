@@ -285,11 +285,12 @@ def test_reference_shaped_caller_runs_on_the_engine(eng):
     assert r.stdout.startswith("ok ")
 
 
-@pytest.mark.parametrize("n", [(1 << 24) + 12345, (1 << 24) + 32 * 1024 * 3 + 7])
+@pytest.mark.parametrize("n", [(1 << 24) + 12345, (1 << 24) + 32 * 1024 * 3 + 7, (1 << 25) + 32 * 1024 * 5 + 9])
 def test_wide_workgroup_queries_on_large_columns(eng, n):
     """Columns of at least 2^24 rows run the query kernel with 2048-word workgroups (two
-    directory tiles each); ragged lengths leave the last workgroup a partial tile or a lone
-    directory tile. Single- and multi-column queries equal numpy, at 128 and 512 bins."""
+    directory tiles each), single-column queries on columns of at least 2^25 rows with
+    4096-word ones; ragged lengths leave the last workgroup a partial tile or fewer
+    directory tiles. Single- and two-column queries equal numpy, at 128 and 512 bins."""
     rng = np.random.default_rng(n % 1000)
     xs = [eng.generate_column(eng.DIST_UNIFORM, 9, 0, n),
           eng.generate_column(eng.DIST_ZIPF, 9, 2, n, 1.1, 2.0 ** 20)]
