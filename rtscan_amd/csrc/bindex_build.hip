@@ -64,6 +64,12 @@ __global__ void k_bin_starts(const CODE *sorted, int64_t n, const int64_t *idx, 
   estarts[b] = sorted[s0 + (int64_t)j * len / sub];
 }
 
+// out[i] = sorted[i * kRankSample] for i < m.
+__global__ void k_rank_samples(const CODE *sorted, int64_t m, CODE *out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < m) out[i] = sorted[i * kRankSample];
+}
+
 // One pass over the column: all bins-1 filter vectors (fv[k] bit i = x_i < estart[k+1],
 // set_fv_val_less, bindex.cpp:553-560) and each row's refine key (its bin if
 // estart[bin] < x, else `bins` = no refine).  One row per lane; a 64-row chunk gives two
@@ -179,6 +185,22 @@ static unsigned blocks_for(int64_t items, int per = kBlock) {
       goto fail;                        \
     }                                   \
   } while (0)
+
+hipError_t rank_samples(rts_bindex_dev *d, hipStream_t s) {
+  const int64_t m = (d->n + kRankSample - 1) / kRankSample;
+  CODE *smp = nullptr;
+  d->rank_samples.assign(m, 0);
+  hipError_t e = hipMalloc((void **)&smp, m * sizeof(CODE));
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_rank_samples, dim3((unsigned)((m + kBlock - 1) / kBlock)), dim3(kBlock), 0, s,
+                       d->sorted_val, m, smp);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(d->rank_samples.data(), smp, m * sizeof(CODE), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(smp);
+  return e;
+}
 
 static void free_dev(rts_bindex_dev *d) {
   if (!d) return;
@@ -398,6 +420,7 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   RTS_TRY(hipMemcpyAsync(d->block_first.data(), bfirst_d, nblocks * sizeof(CODE), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipMemcpyAsync(d->block_last.data(), blast_d, nblocks * sizeof(CODE), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipStreamSynchronize(s));
+  RTS_TRY(rank_samples(d, s));
 
   describe(bx, d, col);
   d = nullptr;
@@ -676,6 +699,7 @@ hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
     e = hipMemcpy(d->boff, d->boff_host.data(), (d->bins + 1) * sizeof(uint32_t), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = launch_tile_directory(d, d->ntiles, kTileRows, d->bdir, ref_stream());
   if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
+  if (e == hipSuccess) e = rank_samples(d, ref_stream());
   if (e == hipSuccess) {
     memset(bx, 0, sizeof(*bx));
     bx->data_max = h.data_max;

@@ -14,6 +14,8 @@
 //                             are [boff[k], boff[k+1])
 //   bdir                      tile directory: bin k's first entry with row >= t * kTileRows
 //                             is bdir[k * (ntiles + 1) + t]
+//   rank_samples              every kRankSample-th sorted value (host): rank_samples[i] =
+//                             sorted_val[i * kRankSample], for counts by rank
 // A bound v of a query (the set L(v) = {x < v}) is then one filter vector plus the rows of
 // one bin's B list with x >= v: L(v) = fv[t(v)] minus {B_t(v) rows with x >= v}, or
 // fv[t(v) - 1] exactly when v is a bin start.  Rows equal to a bin start never need a
@@ -23,6 +25,8 @@
 #pragma once
 
 #include <stdint.h>
+
+#include <algorithm>
 
 #include <vector>
 
@@ -37,6 +41,7 @@ constexpr int kMaxBins = kAreas * kMaxSub;     // 1024
 constexpr int kTileWords = 1024;              // bitmap words per query-kernel tile
 constexpr int kTileRows = kTileWords * 32;    // 32768 rows
 constexpr int kFvStrideAlign = 64;            // words
+constexpr int64_t kRankSample = 256;          // sorted slots per rank sample
 
 }  // namespace rts
 
@@ -61,6 +66,7 @@ struct rts_bindex_dev {
   int64_t area_len[rts::kAreas];
   std::vector<CODE> estarts;         // bins
   std::vector<uint32_t> boff_host;   // bins + 1
+  std::vector<CODE> rank_samples;    // ceil(n / kRankSample)
   std::vector<Area> area_store;      // BinDex::areas
   std::vector<Area> area_gpu_store;  // BinDex::areasInGPU
   std::vector<CODE> block_first, block_last;
@@ -106,6 +112,27 @@ inline BoundSpec bound_spec(const rts_bindex_dev *d, CODE v, bool infinite) {
 // first B-list entry of bin k with row >= t * tile_rows (asynchronous on s).
 hipError_t launch_tile_directory(const rts_bindex_dev *d, int64_t ntiles, int64_t tile_rows, uint32_t *bdir,
                                  hipStream_t s);
+
+// d->rank_samples (host) from the device's sorted values, on s (synchronous).
+hipError_t rank_samples(rts_bindex_dev *d, hipStream_t s);
+
+// Sorted-slot interval [*L, *R] (fewer than kRankSample slots apart) that holds rank(v) =
+// #{x < v}, from the host's samples (v = 2^32 when infinite: rank n).  With c samples below
+// v, slot (c - 1) * kRankSample holds a value < v and slot c * kRankSample (if any) one >= v.
+inline void rank_interval(const rts_bindex_dev *d, CODE v, bool infinite, int64_t *L, int64_t *R) {
+  const std::vector<CODE> &smp = d->rank_samples;
+  if (infinite) {
+    *L = *R = d->n;
+    return;
+  }
+  const int64_t c = std::lower_bound(smp.begin(), smp.end(), v) - smp.begin();
+  if (c == 0) {  // v <= the column's minimum
+    *L = *R = 0;
+    return;
+  }
+  *L = (c - 1) * kRankSample + 1;
+  *R = std::min<int64_t>(c * kRankSample, d->n);
+}
 
 // Refine-list entries of bin k.
 inline int64_t list_entries(const rts_bindex_dev *d, int k) {

@@ -52,14 +52,73 @@ __device__ __forceinline__ uint4 base_words(const BoundDev &b, int64_t w, int64_
   return v;
 }
 
+// Count of a one-column query [a, b] without its bitmap: rank(b + 1) - rank(a) with
+// rank(v) = #{x < v}, each the first slot of the sorted column holding a value >= v.  The
+// host narrows each rank to a slot interval [L, R] of fewer than 256 slots (rank_interval,
+// from its every-256th-slot samples); one wave then counts the slots of both intervals
+// below their bounds: one round of loads.  (Wider intervals take more rounds: each samples
+// 256 evenly spaced slots, and the number below v, a prefix since the values are sorted,
+// narrows the interval 256-fold.)
+struct RankArgs {
+  const CODE *sorted;       // nullptr: no rank count
+  unsigned long long *out;  // the count
+  int64_t L[2], R[2];       // bound 0: rank(a), bound 1: rank(b + 1)
+  CODE v[2];
+};
+
+__device__ void rank_count(const RankArgs &r) {
+  const int lane = threadIdx.x & (kWave - 1);
+  int64_t L[2] = {r.L[0], r.L[1]}, R[2] = {r.R[0], r.R[1]};
+  // every round leaves an interval of fewer than len / 256 + 1 slots, or none: n < 2^31
+  // needs at most four
+  for (int round = 0; round < 6 && (L[0] < R[0] || L[1] < R[1]); round++) {
+    int64_t step[2];
+    CODE x[2][4];
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int64_t len = R[b] - L[b];
+      step[b] = (len + 255) / 256;
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t q = L[b] + (int64_t)(u * kWave + lane) * step[b];
+        x[b][u] = len > 0 && q < R[b] ? r.sorted[q] : 0xFFFFFFFFu;
+      }
+    }
+#pragma unroll
+    for (int b = 0; b < 2; b++) {
+      const int64_t len = R[b] - L[b];
+      if (len <= 0) continue;
+      int64_t c = 0;  // sampled slots holding a value < v: samples 0 .. c-1
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const int64_t q = L[b] + (int64_t)(u * kWave + lane) * step[b];
+        c += __popcll(__ballot(q < R[b] && x[b][u] < r.v[b]));
+      }
+      if (step[b] == 1) {  // every slot sampled: the rank itself
+        L[b] += c;
+        R[b] = L[b];
+      } else if (c == 0) {  // the interval's first value is >= v
+        R[b] = L[b];
+      } else {  // sample c-1 is < v and sample c (or R) is >= v
+        const int64_t nl = L[b] + (c - 1) * step[b] + 1;
+        R[b] = std::min<int64_t>(L[b] + c * step[b], R[b]);
+        L[b] = nl;
+      }
+    }
+  }
+  if (lane == 0) *r.out = (unsigned long long)(L[1] - L[0]);
+}
+
 constexpr int kListU = 2;  // list entries per thread per list per round
 
 // A workgroup owns M directory tiles: TQ = M * kTileWords output words, kPer = TQ / kBlock
 // words per thread (4 or 8, one or two uint4 per filter vector).
+// With rank.sorted set, workgroup 0 writes the count by rank_count (one wave) and the
+// others take the tiles; COUNT (popcount of the stored words) is then false.
 template <int C, int MODE, bool COUNT, int M>
 __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t n, int64_t words,
                                                          int64_t ndir, BITS *out, bool out_vec,
-                                                         CountSink sink) {
+                                                         CountSink sink, RankArgs rank) {
   // One toggle array per column for both bounds: with L(v) = fv ^ t_v the rows below v,
   // L(lo) is inside L(hi + 1), so L(hi + 1) & ~L(lo) = (fv_hi ^ fv_lo) ^ (t_lo ^ t_hi); the two
   // bounds' lists cover different bins (or, both bounds in one bin, a single list) and
@@ -72,7 +131,16 @@ __global__ __launch_bounds__(kBlock) void k_bindex_query(QueryArgs<C> q, int64_t
   const uint32_t last_mask = tail_mask((uint64_t)n);
   const int64_t ntiles = (words + TQ - 1) / TQ;
   uint32_t pc = 0;
-  for (int64_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  int64_t t0 = blockIdx.x, tstride = gridDim.x;
+  if (rank.sorted) {
+    if (blockIdx.x == 0) {
+      if (threadIdx.x < kWave) rank_count(rank);
+      return;
+    }
+    t0 -= 1;
+    tstride -= 1;
+  }
+  for (int64_t t = t0; t < ntiles; t += tstride) {
     // The filter-vector words and the lists' directory words do not depend on the refine:
     // issue them first, so that one round trip covers both and the second round trip is
     // the refine entries.
@@ -205,30 +273,34 @@ constexpr int64_t kWideQueryRows = (int64_t)1 << 24;
 
 template <int C, int MODE, int M>
 static hipError_t launch_query_m(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
-                                 unsigned long long *count, hipStream_t s) {
+                                 unsigned long long *count, const RankArgs &rank, hipStream_t s) {
   const bool vec = ((uintptr_t)out & 15) == 0;
   const int64_t ntiles = (d->words + M * kTileWords - 1) / (M * kTileWords);
   const unsigned grid = (unsigned)std::min<int64_t>(ntiles, (int64_t)device_cu_count() * 32);
-  if (count) {
+  const CountSink none{nullptr, nullptr, nullptr};
+  if (rank.sorted) {  // one more workgroup, for the count
+    hipLaunchKernelGGL((k_bindex_query<C, MODE, false, M>), dim3(grid + 1), dim3(kBlock), 0, s, q, d->n,
+                       d->words, d->ntiles, out, vec, none, rank);
+  } else if (count) {
     const CountSink sink = stream_count_sink(count, s);
     if (!sink.acc) return hipErrorOutOfMemory;
     hipLaunchKernelGGL((k_bindex_query<C, MODE, true, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
-                       d->words, d->ntiles, out, vec, sink);
+                       d->words, d->ntiles, out, vec, sink, rank);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     return launch_count_finalize(sink, grid, s);
   } else {
     hipLaunchKernelGGL((k_bindex_query<C, MODE, false, M>), dim3(grid), dim3(kBlock), 0, s, q, d->n,
-                       d->words, d->ntiles, out, vec, CountSink{nullptr, nullptr, nullptr});
+                       d->words, d->ntiles, out, vec, none, rank);
   }
   return hipGetLastError();
 }
 
 template <int C, int MODE>
 static hipError_t launch_query(const QueryArgs<C> &q, const rts_bindex_dev *d, BITS *out,
-                               unsigned long long *count, hipStream_t s) {
-  if (C <= 2 && d->n >= kWideQueryRows) return launch_query_m<C, MODE, 2>(q, d, out, count, s);
-  return launch_query_m<C, MODE, 1>(q, d, out, count, s);
+                               unsigned long long *count, const RankArgs &rank, hipStream_t s) {
+  if (C <= 2 && d->n >= kWideQueryRows) return launch_query_m<C, MODE, 2>(q, d, out, count, rank, s);
+  return launch_query_m<C, MODE, 1>(q, d, out, count, rank, s);
 }
 
 template <int C>
@@ -249,8 +321,19 @@ static hipError_t query_group(const BinDex *const *bxs, const CODE *lo, const CO
       q.hi[c].dir = nullptr;
     }
   }
-  if (mode == kStoreAnd) return launch_query<C, kStoreAnd>(q, bxs[0]->dev, out, count, s);
-  return launch_query<C, kStoreSet>(q, bxs[0]->dev, out, count, s);
+  RankArgs rank{};
+  if (mode == kStoreAnd) return launch_query<C, kStoreAnd>(q, bxs[0]->dev, out, count, rank, s);
+  if (C == 1 && count) {
+    // one column: the count from the sorted column, beside the bitmap instead of after it
+    const rts_bindex_dev *d = bxs[0]->dev;
+    rank.sorted = d->sorted_val;
+    rank.out = count;
+    rank.v[0] = lo[0];
+    rank.v[1] = hi[0] + 1u;
+    rank_interval(d, rank.v[0], false, &rank.L[0], &rank.R[0]);
+    rank_interval(d, rank.v[1], hi[0] == 0xFFFFFFFFu, &rank.L[1], &rank.R[1]);
+  }
+  return launch_query<C, kStoreSet>(q, bxs[0]->dev, out, count, rank, s);
 }
 
 __global__ void k_zero_count(unsigned long long *p) { *p = 0ull; }

@@ -102,6 +102,52 @@ def test_single_column_query_matches_raw_scan(eng, n):
         ix.free()
 
 
+def engine_bin_starts(ix, sv, bins):
+    """The index's engine bin starts: area a (sorted slots [s0, s0 + len)) cut into
+    bins / 128 equal-count bins, bin j starting at the value in slot s0 + j * len // sub."""
+    ends = ix.area_counts()
+    sub = bins // 128
+    out = []
+    for a in range(128):
+        s0 = 0 if a == 0 else int(ends[a - 1])
+        ln = int(ends[a]) - s0
+        out += [int(sv[s0 + j * ln // sub]) for j in range(sub)]
+    return out
+
+
+@pytest.mark.parametrize("n", [1000, 70001, (1 << 20) + 3])
+def test_one_column_counts_at_every_bin_count(eng, n):
+    """A counted one-column query takes its count from the sorted column (rank(hi + 1) -
+    rank(lo), a search inside each bound's bin) while the other workgroups write the
+    bitmap: counts and bitmaps equal numpy at 128, 256 and 1024 bins, for bounds on, next
+    to and between the engine's bin starts, at the data's ends and beyond them."""
+    rng = np.random.default_rng(n + 7)
+    W = (n + 31) // 32
+    out = eng.DeviceBuffer(4 * W)
+    cnt = eng.DeviceBuffer(8)
+    sets = datasets(eng, n)
+    for name in ("uniform", "zipf15", "small_range", "split_heavy"):
+        x = sets[name]
+        dx = upload(eng, x)
+        sv = np.sort(x)
+        for bins in (128, 256, 1024):
+            ix = eng.BinDexIndex(dx, n, bins=bins)
+            assert ix.bins == bins
+            es = engine_bin_starts(ix, sv, bins)
+            pts = sorted({(e + d) & MAXU for e in rng.choice(es, 12) for d in (-1, 0, 1)})
+            qs = query_bounds(x, ix.start_values(), rng, count=12)
+            qs += [tuple(sorted(int(v) for v in rng.choice(pts, 2))) for _ in range(16)]
+            qs += [(int(sv[0]), int(sv[-1])), (int(sv[-1]), MAXU), (0, int(sv[0]))]
+            for lo, hi in qs:
+                out.fill(0xA5)
+                cnt.fill(0xEE)
+                eng.bindex_query([ix], [lo], [hi], out, cnt)
+                m = interval_mask(x, lo, hi)
+                assert int(cnt.to_numpy(np.uint64, 1)[0]) == int(m.sum()), (name, bins, lo, hi)
+                assert np.array_equal(out.to_numpy(np.uint32, W), pack_rows(m)), (name, bins, lo, hi)
+            ix.free()
+
+
 def test_split_heavy_value_is_exact(eng):
     """The inputs where the reference BinDex misses rows (a value split mid-run across
     areas, queried just above it) are exact here."""
@@ -287,10 +333,10 @@ def test_reference_shaped_caller_runs_on_the_engine(eng):
 
 @pytest.mark.parametrize("n", [(1 << 24) + 12345, (1 << 24) + 32 * 1024 * 3 + 7, (1 << 25) + 32 * 1024 * 5 + 9])
 def test_wide_workgroup_queries_on_large_columns(eng, n):
-    """Columns of at least 2^24 rows run the query kernel with 2048-word workgroups (two
-    directory tiles each), single-column queries on columns of at least 2^25 rows with
-    4096-word ones; ragged lengths leave the last workgroup a partial tile or fewer
-    directory tiles. Single- and two-column queries equal numpy, at 128 and 512 bins."""
+    """Columns of at least 2^24 rows run the one- and two-column query kernels with
+    2048-word workgroups (two directory tiles each); ragged lengths leave the last
+    workgroup a partial tile or one directory tile. Single- and two-column queries equal
+    numpy, at 128 and 512 bins."""
     rng = np.random.default_rng(n % 1000)
     xs = [eng.generate_column(eng.DIST_UNIFORM, 9, 0, n),
           eng.generate_column(eng.DIST_ZIPF, 9, 2, n, 1.1, 2.0 ** 20)]
