@@ -330,6 +330,23 @@ def bench_scan(args, R):
     ix512.free()
 
     progress("BinDex queries timed (128 and 512 bins)")
+    # all 1000 predicates through rts_scan_ranges: the column read once per 64 predicates
+    # (16 calls), the 64 bitmaps of a call written to one 800 MB buffer
+    los, his = [p[0] for p in preds], [p[1] for p in preds]
+    mout = R.DeviceBuffer(4 * W * 64)
+    mcnt = R.DeviceBuffer(8 * len(preds))
+
+    def multi(i, counts=None):
+        for p0 in range(0, len(preds), 64):
+            R.scan_ranges(dx, n, los[p0:p0 + 64], his[p0:p0 + 64], mout, W,
+                          _Slot(counts, 8 * p0) if counts else None, stream)
+
+    ms_multi = timed_loop(R, stream, 2, 1, multi)
+    multi(0, mcnt)
+    stream.synchronize()
+    multi_counts = mcnt.to_numpy(np.uint64, len(preds))
+    multi_alg = (len(preds) + 63) // 64 * 4 * n + len(preds) * 4 * W
+    mout.free()
     cores = cpu_threads()
     first = [pred(i) for i in range(min(steps, len(preds)))]
     seen = set(first)
@@ -337,6 +354,9 @@ def bench_scan(args, R):
     # the port's counts against the GPU's, for the predicates both ran
     gpu_by_pred = {pred(i): int(gpu_counts[i]) for i in range(steps)}
     ok = all(gpu_by_pred.get(p, c) == c for p, c in zip(cpu["order"], cpu["counts"]))
+    multi_by_pred = {p: int(c) for p, c in zip(preds, multi_counts)}
+    multi_ok = all(multi_by_pred[p] == c for p, c in zip(cpu["order"], cpu["counts"])) and \
+        all(multi_by_pred[pred(i)] == int(gpu_counts[i]) for i in range(steps))
     # the reference's own CPU BinDex on the same column and predicates (slot-11 region)
     progress("CPU port timed: %.3g rows/s" % cpu["rows_per_s"])
     ref_b = {"skipped": "--no-cpu"}
@@ -402,6 +422,15 @@ def bench_scan(args, R):
             "bindex_query_512bins_roofline": {"alg_bytes": b512_alg,
                                               "frac": b512_alg / (ms_b512 / 1e3 / steps) / 1e9 / HBM_PEAK_GBS},
             "bindex_device_bytes": ix.device_bytes(),
+            "ranges_in_passes": {
+                "call": "rts_scan_ranges: the 1000 predicates in 16 calls of up to 64, the "
+                        "column read once per call (k_scan_multi), no counts in the timed region",
+                "ms_all_predicates": ms_multi / 2, "us_per_predicate": ms_multi / 2 / len(preds) * 1e3,
+                "row_predicates_per_s": n * len(preds) / (ms_multi / 2 / 1e3),
+                "roofline": {"bound": "hbm", "alg_bytes": multi_alg,
+                             "achieved_GBps": multi_alg / (ms_multi / 2 / 1e3) / 1e9,
+                             "frac": multi_alg / (ms_multi / 2 / 1e3) / 1e9 / HBM_PEAK_GBS},
+                "counts_match": multi_ok},
             # the reference's own compiled code on this host, beside the port above: its
             # raw_scan (one thread, as it runs) and its CPU BinDex query (slot-11 region)
             "cpu_reference_raw_scan": cpu["reference"],
@@ -618,6 +647,14 @@ def bench_config1(R, n=1 << 20, reps=20):
         ok = ok and bool(np.array_equal(got[j], truth)) and int(counts[j]) == oracle.popcount_rows(truth, n)
     us = ms * 1e3 / steps
     alg = 4 * n + 4 * W
+    # the same 64 ranges in one pass over the column (rts_scan_ranges: bitmaps and counts)
+    los, his = [p[0] for p in preds], [p[1] for p in preds]
+    out.fill(0)
+    ms_one = timed_loop(R, stream, reps, 2, lambda i: R.scan_ranges(dx, n, los, his, out, W, cnt, stream))
+    got1 = out.to_numpy(np.uint32, W * len(preds)).reshape(len(preds), W)
+    one_ok = bool(np.array_equal(got1, got)) and bool(np.array_equal(cnt.to_numpy(np.uint64, len(preds)), counts))
+    us_one = ms_one * 1e3 / reps
+    alg_one = 4 * n + 4 * W * len(preds)  # the column once + 64 bitmaps
     threads = cpu_threads()
     done, secs, _ = port_time([x], [[(0, lo, hi)] for lo, hi in preds] * 20, threads, 2.0, 64 * 20)
     ref = reference_raw_scan(x, preds, [], budget_s=1.0, max_preds=64)
@@ -628,6 +665,12 @@ def bench_config1(R, n=1 << 20, reps=20):
                          "frac": alg / (us * 1e-6) / 1e9 / HBM_PEAK_GBS},
             "cpu_baseline": {"port": {"rows_per_s": done * n / secs, "cores": threads, "kind": "port"},
                              "reference_raw_scan": ref},
+            "one_pass": {"call": "rts_scan_ranges, all 64 ranges in one call (k_scan_multi + counts)",
+                         "us_per_call": us_one, "us_per_query": us_one / len(preds),
+                         "rows_per_s": n * len(preds) / (us_one * 1e-6),
+                         "roofline": {"bound": "hbm", "alg_bytes": alg_one,
+                                      "frac": alg_one / (us_one * 1e-6) / 1e9 / HBM_PEAK_GBS},
+                         "equal_to_one_per_launch": one_ok},
             "bitmaps_and_counts_match_oracle": ok}
 
 

@@ -145,6 +145,17 @@ int rts_predicate_interval(enum OPERATOR op, CODE t1, CODE t2, CODE *lo, CODE *h
 hipError_t rts_scan_range(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
                           unsigned long long *count, hipStream_t stream);
 
+/* Many closed-range predicates over one device column, the column read once per 64
+ * predicates (the reference scans one predicate per call, raw_scan bindex.cpp:1851-1892,
+ * and its driver loops over the query file's predicates, bindex_cuda.cpp:1829-1935):
+ * bitmap p, ceil(n/32) words at out + p * out_stride_words, gets the rows with
+ * lo[p] <= x <= hi[p] (lo[p] > hi[p]: no row), tail bits 0; `counts` (device, npred
+ * entries, may be NULL) the number of matching rows of each.  `lo`, `hi` are host arrays
+ * of npred >= 1 entries; out_stride_words >= ceil(n/32) when npred > 1. */
+hipError_t rts_scan_ranges(const CODE *column, uint64_t n, const CODE *lo, const CODE *hi, int npred,
+                           BITS *out, uint64_t out_stride_words, unsigned long long *counts,
+                           hipStream_t stream);
+
 /* As rts_scan_range, but XOR-toggles the matching rows into `out` (raw_scan semantics). */
 hipError_t rts_scan_range_xor(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,
                               hipStream_t stream);

@@ -70,6 +70,7 @@ SIGNATURES = {
                                       ctypes.POINTER(_u32)]),
     "rts_scan_range": (_err, [_vp, _u64, _u32, _u32, _vp, _vp, _vp]),
     "rts_scan_range_xor": (_err, [_vp, _u64, _u32, _u32, _vp, _vp]),
+    "rts_scan_ranges": (_err, [_vp, _u64, _vp, _vp, _int, _vp, _u64, _vp, _vp]),
     "rts_conjunction_scan": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
     "rts_conjunction_scan_noalloc": (_err, [_vp, _vp, _vp, _int, _u64, _vp, _vp, _vp]),
     "rts_bitmap_and_popcount": (_err, [_vp, _int, _u64, _vp, _vp, _vp]),
@@ -312,6 +313,16 @@ def scan_range(col, n, lo, hi, out, count=None, stream=None):
     """Kernel (a): bitmap of rows of device column `col` with lo <= x <= hi."""
     check(lib().rts_scan_range(col.ptr, n, lo, hi, out.ptr, count.ptr if count else None,
                                _sh(stream)), "rts_scan_range")
+
+
+def scan_ranges(col, n, los, his, out, stride_words, counts=None, stream=None):
+    """Many closed ranges over one column, read once per 64 predicates: bitmap p at
+    out + p * stride_words (words), counts[p] (device, optional) its matching rows."""
+    lo = np.ascontiguousarray(los, np.uint32)
+    hi = np.ascontiguousarray(his, np.uint32)
+    check(lib().rts_scan_ranges(col.ptr, n, lo.ctypes.data, hi.ctypes.data, len(lo), out.ptr,
+                                stride_words, counts.ptr if counts else None, _sh(stream)),
+          "rts_scan_ranges")
 
 
 def scan_range_xor(col, n, lo, hi, out, stream=None):

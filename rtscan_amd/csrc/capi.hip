@@ -21,6 +21,8 @@ hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE 
                             hipStream_t s, uint32_t *partials = nullptr);
 hipError_t bitmap_and_popcount(const BITS *const *bms, int nb, uint64_t n, BITS *out,
                                unsigned long long *count, hipStream_t s);
+hipError_t scan_ranges(const CODE *col, uint64_t n, const CODE *lo, const CODE *hi, int np, BITS *out,
+                       uint64_t stride, unsigned long long *counts, hipStream_t s);
 hipError_t bitop(int op, BITS *dst, const BITS *a, const BITS *b, uint64_t words, uint32_t value,
                  hipStream_t s);
 hipError_t flip_rows(BITS *bm, uint64_t rows, hipStream_t s);
@@ -288,6 +290,13 @@ hipError_t rts_scan_range(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS
   const CODE *cols[1] = {column};
   return set_error(scan_conjunction(cols, &lo, &hi, 1, n, out, count, kStoreSet, stream),
                    "rts_scan_range");
+}
+
+hipError_t rts_scan_ranges(const CODE *column, uint64_t n, const CODE *lo, const CODE *hi, int npred,
+                           BITS *out, uint64_t out_stride_words, unsigned long long *counts,
+                           hipStream_t stream) {
+  return set_error(scan_ranges(column, n, lo, hi, npred, out, out_stride_words, counts, stream),
+                   "rts_scan_ranges");
 }
 
 hipError_t rts_scan_range_xor(const CODE *column, uint64_t n, CODE lo, CODE hi, BITS *out,

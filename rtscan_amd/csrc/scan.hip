@@ -210,6 +210,122 @@ __global__ __launch_bounds__(kBlock) void k_and_popcount(BitmapSet s, int nb, ui
   if (COUNT) count_block_partial(pc, sink);
 }
 
+// ------------------------------------------------- many predicates over one column
+
+constexpr int kMultiPreds = 64;  // predicates per pass
+constexpr int kMultiU = 4;       // uint4 loads per lane per tile: 1024 rows, 32 words
+constexpr unsigned kMultiMaxGrid = 4096;
+
+struct MultiPreds {
+  uint32_t lo[kMultiPreds];
+  uint32_t span[kMultiPreds];  // hi - lo
+  uint64_t empty;              // bit p: predicate p matches no row
+};
+
+// Sum over the 64 lanes of a wave by DPP (rows of 16 by quad permutes and rotates, then
+// the row broadcasts), the total read from lane 63: a scalar.
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, true);  // row_ror:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
+// Up to kMultiPreds closed ranges over one column in one pass: a wave reads its tile of
+// 256 * kMultiU rows once (128-bit non-temporal loads, the lane layout of k_scan_tiles),
+// keeps it in registers and, for every predicate p, assembles the tile's 32 words of
+// bitmap p (out + p * stride) and stores them with one instruction (lane l stores word
+// 8 (l % 8) + l / 8 for l % 8 < kMultiU: one 128-byte line).  Rows past the whole tiles
+// (or every row, for a column that is not 16-byte aligned) go one row per lane, 64 rows per
+// wave step.  The predicates are split over gridDim.y groups of `pper` (workgroup y takes
+// [y * pper, y * pper + pper)), so that a short column still fills the device; the groups
+// read the same tiles, the later ones from L2.  COUNT: per-workgroup totals per predicate
+// (DPP wave totals added in LDS) into partials[p * gridDim.x + blockIdx.x], summed by
+// k_multi_counts.
+template <bool COUNT>
+__global__ __launch_bounds__(kBlock) void k_scan_multi(const uint32_t *col, uint64_t n, uint64_t ntiles,
+                                                       MultiPreds mp, int np, int pper, BITS *out,
+                                                       uint64_t stride, uint32_t *partials) {
+  __shared__ uint32_t cnt[kMultiPreds];
+  const int lane = threadIdx.x & (kWave - 1);
+  const int pbeg = blockIdx.y * pper, pend = min(np, pbeg + pper);
+  if (COUNT) {
+    if (threadIdx.x < kMultiPreds) cnt[threadIdx.x] = 0;
+    __syncthreads();
+  }
+  const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
+  const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
+  const uint32_t shift = 28 - 4 * (lane & 7);
+  const int su = lane & 7;                        // the word this lane stores: 8 su + lane / 8
+  const bool storer = su < kMultiU;
+  for (uint64_t t = wave; t < ntiles; t += nwaves) {
+    const uint64_t row0 = t * (uint64_t)(256 * kMultiU);
+    const uint4 *src = reinterpret_cast<const uint4 *>(col + row0) + lane;
+    uint4 v[kMultiU];
+#pragma unroll
+    for (int u = 0; u < kMultiU; u++) v[u] = load_stream(src + 64 * u);
+    BITS *o = out + row0 / 32 + 8 * su + lane / 8;
+    for (int p = pbeg; p < pend; p++) {
+      uint32_t w = 0;
+      if (!((mp.empty >> p) & 1)) {
+        const uint32_t lo = mp.lo[p], span = mp.span[p];
+#pragma unroll
+        for (int u = 0; u < kMultiU; u++) {
+          const uint32_t x = or_lanes8(nibble4(v[u], lo, span) << shift);
+          w = su == u ? x : w;
+        }
+      }
+      if (storer) o[p * stride] = w;
+      if (COUNT) {
+        const uint32_t c = wave_total(storer ? __popc(w) : 0u);
+        if (lane == 0 && c) atomicAdd(&cnt[p], c);
+      }
+    }
+  }
+  // rows past the whole tiles: one row per lane, 64 rows per wave step
+  const uint64_t begin = ntiles * (uint64_t)(256 * kMultiU);
+  const uint64_t nchunks = (n - begin + 63) / 64;
+  for (uint64_t c = wave; c < nchunks; c += nwaves) {
+    const uint64_t r0 = begin + c * 64, r = r0 + lane;
+    const uint32_t x = r < n ? col[r] : 0u;
+    const uint64_t wd = r0 / 32 + (lane & 1);
+    for (int p = pbeg; p < pend; p++) {
+      const bool m = r < n && !((mp.empty >> p) & 1) && (x - mp.lo[p]) <= mp.span[p];
+      const uint64_t b = __ballot(m);
+      const uint32_t word = __brev((lane & 1) ? (uint32_t)(b >> 32) : (uint32_t)b);
+      if (lane < 2 && wd * 32 < n) {
+        out[p * stride + wd] = word;
+        if (COUNT && word) atomicAdd(&cnt[p], (uint32_t)__popc(word));
+      }
+    }
+  }
+  if (COUNT) {
+    __syncthreads();
+    if ((int)threadIdx.x >= pbeg && (int)threadIdx.x < pend)
+      partials[(uint64_t)threadIdx.x * gridDim.x + blockIdx.x] = cnt[threadIdx.x];
+  }
+}
+
+// counts[p] = sum of partials[p * nblocks .. + nblocks); one workgroup per predicate.
+__global__ __launch_bounds__(kBlock) void k_multi_counts(const uint32_t *partials, unsigned nblocks,
+                                                         unsigned long long *counts) {
+  __shared__ unsigned long long part[kBlock / kWave];
+  const uint32_t *src = partials + (uint64_t)blockIdx.x * nblocks;
+  unsigned long long s = 0;
+  for (unsigned i = threadIdx.x; i < nblocks; i += kBlock) s += src[i];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, kWave);
+  if ((threadIdx.x & (kWave - 1)) == 0) part[threadIdx.x / kWave] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / kWave; i++) t += part[i];
+    counts[blockIdx.x] = t;
+  }
+}
+
 // ------------------------------------------------------------------ host launchers
 
 static inline bool aligned16(const void *p) { return ((uintptr_t)p & 15) == 0; }
@@ -332,6 +448,59 @@ hipError_t scan_conjunction(const CODE *const *cols, const CODE *lo, const CODE 
     }
     if (e != hipSuccess) return e;
     done += g;
+  }
+  return hipSuccess;
+}
+
+hipError_t scan_ranges(const CODE *col, uint64_t n, const CODE *lo, const CODE *hi, int np, BITS *out,
+                       uint64_t stride, unsigned long long *counts, hipStream_t s) {
+  if (!col || !lo || !hi || np < 1 || (n && !out)) return hipErrorInvalidValue;
+  const uint64_t words = words_for_rows(n);
+  if (np > 1 && stride < words) return hipErrorInvalidValue;
+  if (n == 0) {
+    if (counts) return hipMemsetAsync(counts, 0, (size_t)np * sizeof(unsigned long long), s);
+    return hipSuccess;
+  }
+  const uint64_t ntiles = aligned16(col) ? n / (256 * kMultiU) : 0;
+  const uint64_t waves = std::max<uint64_t>(ntiles, (n - ntiles * 256 * kMultiU + 63) / 64);
+  const unsigned grid = grid_for(waves, kBlock / kWave, std::min<unsigned>(kMultiMaxGrid, stream_grid_cap() * 2));
+  // predicate groups: enough workgroups for 4 per compute unit, at least 8 predicates each
+  const unsigned want = (unsigned)std::max(1, device_cu_count()) * 4;
+  uint32_t *partials = nullptr;
+  if (counts) {
+    void *pp = nullptr;
+    const hipError_t e = thread_scratch(kScratchMulti, s, (size_t)kMultiPreds * kMultiMaxGrid * sizeof(uint32_t), 0,
+                                        &pp, nullptr);
+    if (e != hipSuccess) return e;
+    partials = static_cast<uint32_t *>(pp);
+  }
+  for (int p0 = 0; p0 < np; p0 += kMultiPreds) {  // one pass over the column per 64 predicates
+    const int k = std::min(kMultiPreds, np - p0);
+    MultiPreds mp{};
+    for (int p = 0; p < k; p++) {
+      if (lo[p0 + p] > hi[p0 + p]) {
+        mp.empty |= 1ull << p;
+      } else {
+        mp.lo[p] = lo[p0 + p];
+        mp.span[p] = hi[p0 + p] - lo[p0 + p];
+      }
+    }
+    BITS *o = out + (uint64_t)p0 * stride;
+    const int groups = (int)std::max<unsigned>(1, std::min<unsigned>((k + 7) / 8, (want + grid - 1) / grid));
+    const int pper = (k + groups - 1) / groups;
+    const dim3 g2(grid, (unsigned)((k + pper - 1) / pper));
+    if (counts) {
+      hipLaunchKernelGGL(k_scan_multi<true>, g2, dim3(kBlock), 0, s, col, n, ntiles, mp, k, pper, o, stride,
+                         partials);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+      hipLaunchKernelGGL(k_multi_counts, dim3(k), dim3(kBlock), 0, s, partials, grid, counts + p0);
+    } else {
+      hipLaunchKernelGGL(k_scan_multi<false>, g2, dim3(kBlock), 0, s, col, n, ntiles, mp, k, pper, o, stride,
+                         nullptr);
+    }
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
   }
   return hipSuccess;
 }

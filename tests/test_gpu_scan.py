@@ -60,8 +60,77 @@ def test_config1_plain_scan(eng):
         assert read_count(eng, cnt) == O.popcount_rows(truth, n)
 
 
+def test_config1_ranges_in_one_pass(eng):
+    """The same 64 ranges through rts_scan_ranges (the column read once for all of them):
+    every bitmap and count equal the oracle's raw_scan."""
+    n = 1 << 20
+    x = eng.generate_column(eng.DIST_UNIFORM, 1, 0, n)
+    dx = upload(eng, x)
+    W = n // 32
+    preds = config1_predicates(64)
+    out = eng.DeviceBuffer(4 * W * len(preds))
+    cnt = eng.DeviceBuffer(8 * len(preds))
+    eng.scan_ranges(dx, n, [p[0] for p in preds], [p[1] for p in preds], out, W, cnt)
+    got = out.to_numpy(np.uint32, W * len(preds)).reshape(len(preds), W)
+    counts = cnt.to_numpy(np.uint64, len(preds))
+    for i, (lo, hi) in enumerate(preds):
+        op, t1, t2 = ref_predicate(lo, hi)
+        truth = O.raw_scan(x, op, t1, t2)
+        assert np.array_equal(got[i], truth), (lo, hi)
+        assert int(counts[i]) == O.popcount_rows(truth, n)
+
+
 SIZES = [1, 2, 31, 32, 33, 63, 64, 65, 255, 256, 257, 1023, 1024, 1025, 2047, 2048, 4097,
          65536 + 77, 300001]
+
+
+@pytest.mark.parametrize("n", [1, 33, 1023, 1024, 1025, 4097, 65536 + 77, 300001])
+@pytest.mark.parametrize("npred", [1, 7, 64, 150])
+def test_scan_ranges_sizes_passes_and_padding(eng, n, npred):
+    """Whole tiles and ragged tails, one pass (<= 64 predicates) and several (150), empty
+    ranges (lo > hi) and the full range; words between bitmaps (stride padding) untouched."""
+    rng = np.random.default_rng(n * 1000 + npred)
+    x = rng.integers(0, 2 ** 32, n, dtype=np.uint32)
+    x[: min(n, 3)] = [0, MAXU, 7][: min(n, 3)]
+    dx = upload(eng, x)
+    W = (n + 31) // 32
+    stride = W + 5
+    fixed = [(0, MAXU), (0, 0), (MAXU, MAXU), (9, 3), (int(x[n // 2]), int(x[n // 2]))]
+    preds = []
+    for i in range(npred):
+        if i < len(fixed):
+            preds.append(fixed[i])
+        else:
+            a, b = sorted(int(v) for v in rng.integers(0, 2 ** 32, 2))
+            preds.append((a, b))
+    out = eng.DeviceBuffer(4 * stride * npred)
+    out.fill(0x5A)
+    cnt = eng.DeviceBuffer(8 * npred)
+    eng.scan_ranges(dx, n, [p[0] for p in preds], [p[1] for p in preds], out, stride, cnt)
+    got = out.to_numpy(np.uint32, stride * npred).reshape(npred, stride)
+    counts = cnt.to_numpy(np.uint64, npred)
+    for i, (lo, hi) in enumerate(preds):
+        m = interval_mask(x, lo, hi) if lo <= hi else np.zeros(n, bool)
+        assert np.array_equal(got[i, :W], pack_rows(m)), (n, i, lo, hi)
+        assert np.all(got[i, W:] == 0x5A5A5A5A), (n, i)
+        assert int(counts[i]) == int(m.sum()), (n, i, lo, hi)
+
+
+@pytest.mark.parametrize("offset_rows", [1, 3])
+def test_scan_ranges_unaligned_column_without_counts(eng, offset_rows):
+    n = 70000 + offset_rows
+    rng = np.random.default_rng(offset_rows + 40)
+    x = rng.integers(0, 1000, n + 8, dtype=np.uint32)
+    dx = upload(eng, x)
+    view = View(dx, 4 * offset_rows)
+    xs = x[offset_rows: offset_rows + n]
+    W = (n + 31) // 32
+    preds = [(100, 499), (0, 0), (999, 999), (1, 998)]
+    out = eng.DeviceBuffer(4 * W * len(preds))
+    eng.scan_ranges(view, n, [p[0] for p in preds], [p[1] for p in preds], out, W)
+    got = out.to_numpy(np.uint32, W * len(preds)).reshape(len(preds), W)
+    for i, (lo, hi) in enumerate(preds):
+        assert np.array_equal(got[i], pack_rows(interval_mask(xs, lo, hi))), (i, lo, hi)
 
 
 @pytest.mark.parametrize("n", SIZES)
