@@ -40,6 +40,18 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// Sum over the 64 lanes of a wave by DPP (rows of 16 by quad permutes and rotates, then
+// the row broadcasts), the total read from lane 63: a scalar.
+__device__ __forceinline__ uint32_t wave_total(uint32_t v) {
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, true);   // quad_perm [1,0,3,2]
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, true);   // quad_perm [2,3,0,1]
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, true);  // row_ror:4
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, true);  // row_ror:8
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x142, 0xA, 0xF, false); // row_bcast:15
+  v += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x143, 0xC, 0xF, false); // row_bcast:31
+  return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
+}
+
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 // 16-byte streaming load (non-temporal: the column is read once per scan).
@@ -92,16 +104,17 @@ struct CountSink {
 };
 
 // Block-wide sum of the threads' counts into the launch's count.  Every block of the grid
-// must call it exactly once; the grid must not exceed kMaxCountBlocks blocks.
+// (of kBlock threads) must call it exactly once; the grid must not exceed kMaxCountBlocks
+// blocks.
 __device__ __forceinline__ void count_block_partial(uint32_t pc, CountSink sink) {
   __shared__ uint32_t part[kBlock / kWave];
-  pc = wave_sum(pc);
+  pc = wave_total(pc);
   const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
   if (lane == 0) part[wv] = pc;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t s = 0;
-    for (int i = 0; i < (int)(blockDim.x / kWave); i++) s += part[i];
+    for (int i = 0; i < kBlock / kWave; i++) s += part[i];
     if (!sink.acc) {
       sink.partials[blockIdx.x] = s;
       return;
