@@ -334,8 +334,9 @@ static unsigned scan_grid_cap() { return (unsigned)std::min<int>(kMaxCountBlocks
 
 // A counted scan runs 32 workgroups per compute unit over the tiles instead: each workgroup
 // then ends with one count reduction and one accumulator add for ~3 of the uncounted grid's
-// workgroups (1e8-row counted scan 68.9 -> 65.2 us, 4-column conjunction 262 -> 254 us;
-// 8 per CU measured 67.2 and 257).
+// workgroups.  On uniform columns scanned back to back: 1e8-row counted scan 68.9 -> 65.2
+// us, 4-column conjunction 262 -> 254 us (8 per CU: 67.2 and 257); on the mixed-distribution
+// 1e8 x 4 table, queries interleaved with other paths, the conjunctions were unchanged.
 static unsigned counted_scan_grid_cap() { return (unsigned)std::min<int>(kMaxCountBlocks, std::max(1, device_cu_count()) * 32); }
 
 template <int OP>
