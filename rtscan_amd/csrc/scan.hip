@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kBlock) void k_and_popcount(BitmapSet s, int nb, ui
 // ------------------------------------------------- many predicates over one column
 
 constexpr int kMultiPreds = 64;  // predicates per pass
-constexpr int kMultiU = 4;       // uint4 loads per lane per tile: 1024 rows, 32 words
+constexpr int kMultiLaneRows = 32;                      // rows per lane: one bitmap word
+constexpr int kMultiTileRows = kWave * kMultiLaneRows;  // 2048 rows per wave tile
 constexpr unsigned kMultiMaxGrid = 4096;
 
 struct MultiPreds {
@@ -222,11 +223,12 @@ struct MultiPreds {
   uint64_t empty;              // bit p: predicate p matches no row
 };
 
-// Up to kMultiPreds closed ranges over one column in one pass: a wave reads its tile of
-// 256 * kMultiU rows once (128-bit non-temporal loads, the lane layout of k_scan_tiles),
-// keeps it in registers and, for every predicate p, assembles the tile's 32 words of
-// bitmap p (out + p * stride) and stores them with one instruction (lane l stores word
-// 8 (l % 8) + l / 8 for l % 8 < kMultiU: one 128-byte line).  Rows past the whole tiles
+// Up to kMultiPreds closed ranges over one column in one pass.  A wave takes a tile of
+// 2048 rows, lane l the 32 rows of bitmap word l (eight 16-byte loads of its own 128 bytes;
+// the wave's loads cover the tile's 8 KB once), kept in registers; for every predicate p
+// the lane shifts its rows' match bits into one MSB-first word (subtract, compare, add
+// with carry per row: no cross-lane assembly) and the wave stores its 64 consecutive
+// words of bitmap p (out + p * stride) with one instruction.  Rows past the whole tiles
 // (or every row, for a column that is not 16-byte aligned) go one row per lane, 64 rows per
 // wave step.  The predicates are split over gridDim.y groups of `pper` (workgroup y takes
 // [y * pper, y * pper + pper)), so that a short column still fills the device; the groups
@@ -246,35 +248,35 @@ __global__ __launch_bounds__(kBlock) void k_scan_multi(const uint32_t *col, uint
   }
   const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   const uint64_t nwaves = (uint64_t)gridDim.x * (kBlock / kWave);
-  const uint32_t shift = 28 - 4 * (lane & 7);
-  const int su = lane & 7;                        // the word this lane stores: 8 su + lane / 8
-  const bool storer = su < kMultiU;
   for (uint64_t t = wave; t < ntiles; t += nwaves) {
-    const uint64_t row0 = t * (uint64_t)(256 * kMultiU);
-    const uint4 *src = reinterpret_cast<const uint4 *>(col + row0) + lane;
-    uint4 v[kMultiU];
+    const uint64_t row0 = t * (uint64_t)kMultiTileRows;
+    const uint4 *src = reinterpret_cast<const uint4 *>(col + row0 + (uint64_t)kMultiLaneRows * lane);
+    uint32_t x[kMultiLaneRows];
 #pragma unroll
-    for (int u = 0; u < kMultiU; u++) v[u] = load_stream(src + 64 * u);
-    BITS *o = out + row0 / 32 + 8 * su + lane / 8;
+    for (int k = 0; k < kMultiLaneRows / 4; k++) {
+      const uint4 v = src[k];
+      x[4 * k] = v.x;
+      x[4 * k + 1] = v.y;
+      x[4 * k + 2] = v.z;
+      x[4 * k + 3] = v.w;
+    }
+    BITS *o = out + row0 / 32 + lane;
     for (int p = pbeg; p < pend; p++) {
       uint32_t w = 0;
       if (!((mp.empty >> p) & 1)) {
         const uint32_t lo = mp.lo[p], span = mp.span[p];
 #pragma unroll
-        for (int u = 0; u < kMultiU; u++) {
-          const uint32_t x = or_lanes8(nibble4(v[u], lo, span) << shift);
-          w = su == u ? x : w;
-        }
+        for (int r = 0; r < kMultiLaneRows; r++) w = w + w + (uint32_t)(x[r] - lo <= span);
       }
-      if (storer) o[p * stride] = w;
+      o[p * stride] = w;
       if (COUNT) {
-        const uint32_t c = wave_total(storer ? __popc(w) : 0u);
+        const uint32_t c = wave_total(__popc(w));
         if (lane == 0 && c) atomicAdd(&cnt[p], c);
       }
     }
   }
   // rows past the whole tiles: one row per lane, 64 rows per wave step
-  const uint64_t begin = ntiles * (uint64_t)(256 * kMultiU);
+  const uint64_t begin = ntiles * (uint64_t)kMultiTileRows;
   const uint64_t nchunks = (n - begin + 63) / 64;
   for (uint64_t c = wave; c < nchunks; c += nwaves) {
     const uint64_t r0 = begin + c * 64, r = r0 + lane;
@@ -456,8 +458,8 @@ hipError_t scan_ranges(const CODE *col, uint64_t n, const CODE *lo, const CODE *
     if (counts) return hipMemsetAsync(counts, 0, (size_t)np * sizeof(unsigned long long), s);
     return hipSuccess;
   }
-  const uint64_t ntiles = aligned16(col) ? n / (256 * kMultiU) : 0;
-  const uint64_t waves = std::max<uint64_t>(ntiles, (n - ntiles * 256 * kMultiU + 63) / 64);
+  const uint64_t ntiles = aligned16(col) ? n / kMultiTileRows : 0;
+  const uint64_t waves = std::max<uint64_t>(ntiles, (n - ntiles * kMultiTileRows + 63) / 64);
   const unsigned grid = grid_for(waves, kBlock / kWave, std::min<unsigned>(kMultiMaxGrid, stream_grid_cap() * 2));
   // predicate groups: enough workgroups for 4 per compute unit, at least 8 predicates each
   const unsigned want = (unsigned)std::max(1, device_cu_count()) * 4;
