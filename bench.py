@@ -1063,7 +1063,8 @@ def main():
     CPU_BASELINES = not args.no_cpu
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this engine runs on one GPU only; --gpus %d is not supported "
-              "(row sharding across GPUs is described in DESIGN.md, not run)" % args.gpus,
+              "(row sharding across GPUs is designed, not built: every GPU would take a "
+              "contiguous range of rows and its own index)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
     import rtscan_amd as R
