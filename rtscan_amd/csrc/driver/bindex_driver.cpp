@@ -65,7 +65,7 @@ int main(int argc, char **argv) {
     switch (opt) {
       case 'h':
         printf("Usage: %s [-f <input-file>] [-b <columns>] [-p <scan-cmd-file>] [-n <rows>] [-s <seed>] "
-               "[-m single|batch] [-e]\n", argv[0]);
+               "[-m single|batch|ranges] [-e]\n", argv[0]);
         return 0;
       case 'f': data_path = optarg; break;
       case 'b': ncols = atoi(optarg); break;
@@ -79,8 +79,12 @@ int main(int argc, char **argv) {
         return 2;
     }
   }
-  if (ncols < 1 || ncols > RTS_BATCH_MAX_COLS || (mode != "single" && mode != "batch")) {
-    fprintf(stderr, "[ERROR] need 1 <= -b <= %d and -m single|batch\n", RTS_BATCH_MAX_COLS);
+  if (ncols < 1 || ncols > RTS_BATCH_MAX_COLS || (mode != "single" && mode != "batch" && mode != "ranges")) {
+    fprintf(stderr, "[ERROR] need 1 <= -b <= %d and -m single|batch|ranges\n", RTS_BATCH_MAX_COLS);
+    return 2;
+  }
+  if (mode == "ranges" && ncols != 1) {  // rts_scan_ranges: many predicates over one column
+    fprintf(stderr, "[ERROR] -m ranges runs one-column query files (-b 1)\n");
     return 2;
   }
   if (!data_path.empty() && n == 0) {
@@ -190,7 +194,8 @@ int main(int argc, char **argv) {
   const size_t words = (size_t)(n + 31) / 32, stride = (words + 3) / 4 * 4;
   BITS *dev_out = nullptr, *h_result = nullptr;
   unsigned long long *dev_count = nullptr;
-  const int nout = mode == "batch" ? (nq > 0 ? nq : 1) : 1;
+  const bool many = mode == "batch" || mode == "ranges";  // all queries in one call
+  const int nout = many ? (nq > 0 ? nq : 1) : 1;
   if (rts_malloc((void **)&dev_out, stride * 4 * nout) != hipSuccess ||
       rts_malloc((void **)&dev_count, 8 * nout) != hipSuccess ||
       rts_host_alloc((void **)&h_result, stride * 4) != hipSuccess)
@@ -247,6 +252,22 @@ int main(int argc, char **argv) {
     batch_ms = ms_since(tb);
     printf("[Time] batched launch of %d queries: %.3f ms\n", np, batch_ms);
   }
+  if (mode == "ranges" && !plan_specs.empty()) {
+    // every query's range on the one column, the column read once per 64 of them
+    const int np = (int)plan_specs.size();
+    std::vector<CODE> lo(np), hi(np);
+    for (int i = 0; i < np; i++) {
+      lo[i] = plan_specs[i].lo[0];
+      hi[i] = plan_specs[i].hi[0];
+    }
+    auto tb = std::chrono::steady_clock::now();
+    if (rts_scan_ranges(bx[0].rawDataInGPU, (uint64_t)n, lo.data(), hi.data(), np, dev_out, (uint64_t)stride,
+                        dev_count, stream) != hipSuccess ||
+        rts_stream_synchronize(stream) != hipSuccess)
+      return fail("rts_scan_ranges");
+    batch_ms = ms_since(tb);
+    printf("[Time] rts_scan_ranges of %d predicates: %.3f ms\n", np, batch_ms);
+  }
 
   std::vector<BITS> check(words), colbm(words);
   int mismatches = 0;
@@ -267,7 +288,7 @@ int main(int argc, char **argv) {
     BITS *res = dev_out;
     double scan_ms = 0;
     if (has_list[q]) {
-      res = mode == "batch" ? dev_out + (size_t)(plan_specs.size() % nout) * stride : dev_out;
+      res = many ? dev_out + (size_t)(plan_specs.size() % nout) * stride : dev_out;
       auto ts = std::chrono::steady_clock::now();
       if (list_query(q, res, dev_count) != hipSuccess || rts_stream_synchronize(stream) != hipSuccess)
         return fail("target-list query");

@@ -42,6 +42,20 @@ def test_reference_query_files(eng, qfile, cols, mode):
     assert m and m.group(1) == m.group(2) and int(m.group(2)) == len(h)
 
 
+def test_one_column_query_file_in_one_ranges_call(eng):
+    """-m ranges: a one-column query file's predicates in one rts_scan_ranges call (the
+    reference driver scans them one by one); every bitmap equals raw_scan's."""
+    rc, out, err = run(["-b", "1", "-p", os.path.join(GOLDEN, "scan_cmd_32_1c.txt"), "-n", "1000003", "-s", "7",
+                        "-m", "ranges"])
+    assert rc == 0, (rc, err[-2000:], out[-2000:])
+    h = hits(out)
+    assert len(h) >= 10 and all(a == b for a, b in h)
+    assert "[Time] rts_scan_ranges of %d predicates" % len(h) in out
+    assert "[CHECK] %d of %d queries matched" % (len(h), len(h)) in out
+    rc, _, err = run(["-b", "3", "-p", os.path.join(GOLDEN, "scan_cmd_32_3c.txt"), "-n", "1000003", "-m", "ranges"])
+    assert rc == 2 and "one-column" in err
+
+
 def test_data_file_and_counts(eng, tmp_path):
     n, cols = 300_007, 2
     data = [R.generate_column(R.DIST_UNIFORM, 3, 0, n),
