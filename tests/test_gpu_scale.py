@@ -50,6 +50,28 @@ def test_scan_matches_oracle_digest_and_count(eng, big):
     cnt.free()
 
 
+def test_scan_ranges_equal_scan(eng, big):
+    """rts_scan_ranges over the 1e9-row column: every bitmap and count equal the one-range
+    scan's (tiles of 2048 rows, one word per lane, plus the ragged tail rows)."""
+    x, d, preds = big
+    k = len(preds)
+    stride = W + 3
+    out = eng.DeviceBuffer(4 * stride * k)
+    counts = eng.DeviceBuffer(8 * k)
+    print("[scale] rts_scan_ranges of %d predicates" % k, flush=True)
+    eng.scan_ranges(d, N, [p[0] for p in preds], [p[1] for p in preds], out, stride, counts)
+    got = [int(c) for c in counts.to_numpy(np.uint64, k)]
+    one = eng.DeviceBuffer(4 * W)
+    cnt = eng.DeviceBuffer(8)
+    for i, (lo, hi) in enumerate(preds):
+        eng.scan_range(d, N, lo, hi, one, cnt)
+        view = types.SimpleNamespace(ptr=out.ptr + 4 * stride * i)
+        assert eng.bitmap_digest(view, N) == eng.bitmap_digest(one, N), (lo, hi)
+        assert got[i] == int(cnt.to_numpy(np.uint64, 1)[0]), (lo, hi)
+    for b in (out, counts, one, cnt):
+        b.free()
+
+
 def test_bindex_query_and_batch_equal_scan(eng, big):
     x, d, preds = big
     print("[scale] BinDex build", flush=True)
