@@ -176,6 +176,15 @@ def test_entry_points_reject_bad_arguments_before_any_device_work():
     assert L.rts_encode_column(None, 10, 0, 10, None, None, None, None, None, ctypes.byref(m),
                                None) == HIP_ERROR_INVALID_VALUE
     assert L.rts_scan_range(None, 10, 0, 5, None, None, None) == HIP_ERROR_INVALID_VALUE
+    # many ranges per pass: no column, no predicates, bitmaps overlapping (stride < words)
+    lo = (ctypes.c_uint32 * 2)(1, 2)
+    hi = (ctypes.c_uint32 * 2)(5, 6)
+    fake = ctypes.c_void_p(256)  # never dereferenced: the checks come first
+    assert L.rts_scan_ranges(None, 1000, lo, hi, 2, fake, 32, None, None) == HIP_ERROR_INVALID_VALUE
+    assert L.rts_scan_ranges(fake, 1000, lo, hi, 0, fake, 32, None, None) == HIP_ERROR_INVALID_VALUE
+    assert L.rts_scan_ranges(fake, 1000, None, hi, 2, fake, 32, None, None) == HIP_ERROR_INVALID_VALUE
+    assert L.rts_scan_ranges(fake, 1000, lo, hi, 2, fake, 31, None, None) == HIP_ERROR_INVALID_VALUE
+    assert b"rts_scan_ranges" in L.rts_last_error()
 
 
 def test_batch_default_config_is_the_documented_shape():
