@@ -140,13 +140,6 @@ CountSink stream_count_sink(unsigned long long *out, hipStream_t stream) {
   return CountSink{out, nullptr, static_cast<unsigned long long *>(p)};
 }
 
-uint32_t *stream_count_partials(hipStream_t stream) {
-  void *p = nullptr;
-  if (thread_scratch(kScratchCountPartials, stream, kMaxCountBlocks * sizeof(uint32_t), 0, &p, nullptr) != hipSuccess)
-    return nullptr;
-  return static_cast<uint32_t *>(p);
-}
-
 // One workgroup of 1024 threads; each thread keeps 8 independent loads in flight so the
 // sum over tens of thousands of partials costs a few memory latencies, not one per slot.
 constexpr int kFinalizeThreads = 1024;

@@ -148,12 +148,10 @@ inline hipStream_t ref_stream() { return hipStreamPerThread; }
 // Scratch of (current device, stream, calling host thread), allocated on first use and
 // kept by the calling thread (a bounded number per thread, freed when the thread exits).
 // Not for use under graph capture: the first use on a stream allocates.
-enum ScratchKind : int { kScratchCountPartials = 0, kScratchCheck = 1, kScratchCountAcc = 2, kScratchMulti = 3 };
+enum ScratchKind : int { kScratchCheck = 1, kScratchCountAcc = 2, kScratchMulti = 3 };
 hipError_t thread_scratch(int kind, hipStream_t stream, size_t dev_bytes, size_t host_bytes, void **dptr,
                           void **hptr, bool *created = nullptr);
 size_t thread_scratch_entries();
-// Count partials (kMaxCountBlocks slots) of (device, stream, calling thread).
-uint32_t *stream_count_partials(hipStream_t stream);
 // The count sink of a counted launch on `stream` writing *out: the calling thread's
 // accumulators for the stream (zeroed when first made, on the stream), or sink.acc ==
 // nullptr when they cannot be had (the launch then fails with hipErrorOutOfMemory).
