@@ -30,6 +30,7 @@
 #include <algorithm>
 #include <array>
 #include <map>
+#include <set>
 #include <vector>
 
 #include "batch.h"
@@ -959,65 +960,94 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
     }
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
   }
-  // 3. groups: consecutive queries (in that order) while their slots fit the LDS of one
-  //    tile and the configured group size, each group one self-contained record
+  // 3. groups, each one self-contained record of queries whose slots fit the LDS of one
+  //    tile, at most the configured group size.  Work items, and their fixed costs, scale
+  //    with the group count, so groups are packed full: first fit by decreasing slot count
+  //    (4 | 3 + 1 | 2 + 2 for four slots), each size class taken in that order, and the
+  //    groups run in the order of their first query.  Partners are then paired in order
+  //    too (the i-th 3-column query with the i-th 1-column one), so neighbouring groups
+  //    still share filter vectors in the locality order.
   const int slot_cap = kBatchLdsWords / 512 * p->cfg.threads / p->cfg.tile_words;
   const int64_t per = p->ntiles + 1;
-  std::vector<BatchGroup> groups;
-  for (int oi = 0; oi < nq;) {
-    int q = order[oi];
-    BatchGroup g;
-    memset(&g, 0, sizeof(g));
-    for (BatchList &L : g.list) L.dir = p->d_zero_dir;
-    g.q0 = oi;
-    while (oi < nq && g.nq < p->cfg.queries_per_group) {
-      q = order[oi];
-      const int need = qempty[q] ? 0 : qcols[q];
-      if (g.nslots + need > slot_cap) break;
-      g.qslot[g.nq] = (uint8_t)g.nslots;
-      for (int c = 0; c < need; c++) {
-        const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
-        BatchSlot &S = g.slot[g.nslots];
-        if (P.direct) {
-          S.col = p->columns[P.col];
-          S.dlo = P.dlo;
-          S.dspan = P.dspan;
+  auto add_query = [&](BatchGroup &g, int q) {
+    const int need = qempty[q] ? 0 : qcols[q];
+    g.qslot[g.nq] = (uint8_t)g.nslots;
+    for (int c = 0; c < need; c++) {
+      const PlannedCol &P = pc[(size_t)q * kBatchMaxCols + c];
+      BatchSlot &S = g.slot[g.nslots];
+      if (P.direct) {
+        S.col = p->columns[P.col];
+        S.dlo = P.dlo;
+        S.dspan = P.dspan;
+      } else {
+        S.fvhi = P.hi.fv;
+        S.fvlo = P.lo.fv;
+        S.hibase = P.hi.base;
+        S.lobase = P.lo.base;
+        const rts_bindex_dev *d = p->index[P.col]->dev;
+        if (P.narrow) {  // t_lo ^ t_hi = the bin's rows with lo <= x <= hi: one pass
+          BatchList &L = g.list[g.nlists++];
+          L.dir = p->dir_base[P.col] + (size_t)P.lo.list * per;
+          L.bent = d->bent;
+          L.thr = P.lo_v;
+          L.span = P.hi_v - P.lo_v;
+          L.slot = (uint32_t)g.nslots;
         } else {
-          S.fvhi = P.hi.fv;
-          S.fvlo = P.lo.fv;
-          S.hibase = P.hi.base;
-          S.lobase = P.lo.base;
-          const rts_bindex_dev *d = p->index[P.col]->dev;
-          if (P.narrow) {  // t_lo ^ t_hi = the bin's rows with lo <= x <= hi: one pass
+          for (const PlannedBound *B : {&P.lo, &P.hi}) {
+            if (B->list < 0) continue;
             BatchList &L = g.list[g.nlists++];
-            L.dir = p->dir_base[P.col] + (size_t)P.lo.list * per;
+            L.dir = p->dir_base[P.col] + (size_t)B->list * per;
             L.bent = d->bent;
-            L.thr = P.lo_v;
-            L.span = P.hi_v - P.lo_v;
+            L.thr = B->thr;
+            L.span = 0xFFFFFFFFu - B->thr;  // value >= thr
             L.slot = (uint32_t)g.nslots;
-          } else {
-            for (const PlannedBound *B : {&P.lo, &P.hi}) {
-              if (B->list < 0) continue;
-              BatchList &L = g.list[g.nlists++];
-              L.dir = p->dir_base[P.col] + (size_t)B->list * per;
-              L.bent = d->bent;
-              L.thr = B->thr;
-              L.span = 0xFFFFFFFFu - B->thr;  // value >= thr
-              L.slot = (uint32_t)g.nslots;
-            }
           }
         }
-        g.nslots++;
       }
-      BatchQuery &Q = g.query[g.nq];
-      Q.out = outs ? outs[q] : nullptr;
-      Q.empty = qempty[q];
-      Q.id = q;
-      g.nq++;
-      oi++;
+      g.nslots++;
     }
+    BatchQuery &Q = g.query[g.nq];
+    Q.out = outs ? outs[q] : nullptr;
+    Q.empty = qempty[q];
+    Q.id = q;
+    g.nq++;
     g.qslot[g.nq] = (uint8_t)g.nslots;
-    groups.push_back(g);
+  };
+  std::vector<int> byneed(nq);  // positions in the order, by decreasing slot count
+  for (int i = 0; i < nq; i++) byneed[i] = i;
+  auto need_at = [&](int oi) { return qempty[order[oi]] ? 0 : qcols[order[oi]]; };
+  std::stable_sort(byneed.begin(), byneed.end(), [&](int a, int b) { return need_at(a) > need_at(b); });
+  std::vector<BatchGroup> cg;
+  cg.reserve(nq);
+  std::vector<int> lead;                         // order position of each group's first query
+  std::vector<std::set<int>> open(slot_cap + 1);  // groups with room, by free slots
+  for (int oi : byneed) {
+    const int need = need_at(oi);
+    int k = -1, kf = 0;
+    for (int f = std::max(need, 1); f <= slot_cap; f++)  // first fit: the earliest group with room
+      if (!open[f].empty() && (k < 0 || *open[f].begin() < k)) k = *open[f].begin(), kf = f;
+    if (k < 0) {
+      k = (int)cg.size();
+      cg.emplace_back();
+      BatchGroup &g = cg.back();
+      memset(&g, 0, sizeof(g));
+      for (BatchList &L : g.list) L.dir = p->d_zero_dir;
+      lead.push_back(oi);
+    } else {
+      open[kf].erase(k);
+    }
+    add_query(cg[k], order[oi]);
+    const int free_slots = slot_cap - cg[k].nslots;
+    if (free_slots > 0 && cg[k].nq < p->cfg.queries_per_group) open[free_slots].insert(k);
+  }
+  std::vector<int> gi(cg.size());
+  for (size_t k = 0; k < cg.size(); k++) gi[k] = (int)k;
+  std::sort(gi.begin(), gi.end(), [&](int a, int b) { return lead[a] < lead[b]; });
+  std::vector<BatchGroup> groups;
+  groups.reserve(cg.size());
+  for (int k : gi) {
+    cg[k].q0 = lead[k];
+    groups.push_back(cg[k]);
   }
   p->has_outs = outs != nullptr;
   p->outs.assign(outs ? outs : (BITS *const *)nullptr, outs ? outs + nq : (BITS *const *)nullptr);

@@ -195,6 +195,36 @@ def test_group_size_change_regroups_and_cache_hits_apply_the_cached_shape(eng, t
     check_results(eng, cols, queries, out, counts, stride)
 
 
+@pytest.mark.parametrize("threads,order", [(256, 1), (256, 0), (512, 1)])
+def test_groups_are_packed_to_the_fewest_work_items(eng, threads, order):
+    """Queries of 1..4 columns, shuffled, none empty: first fit by decreasing column count
+    packs them into ceil(columns / slots) groups when the 1- and 3-column queries pair up
+    (4 | 3 + 1 | 2 + 2 ... for slots = 4 at 2048-word tiles and 256 threads, 8 at 512), and
+    every query still gets its own exact bitmap and count."""
+    n = (1 << 18) + 77
+    cols, dcols, idx, sc = make_table(eng, n, seed=29)
+    rng = np.random.default_rng(threads + order)
+    shape = [4] * 9 + [3] * 14 + [2] * 11 + [1] * 14
+    rng.shuffle(shape)
+    queries = []
+    for k in shape:
+        q = random_queries(rng, sc, 1, with_edges=False)[0]
+        while len(q) != k:
+            q = random_queries(rng, sc, 1, with_edges=False)[0]
+        queries.append(q)
+    nq = len(queries)
+    slots = 32 * threads // 2048
+    plan = eng.BatchPlan(dcols, idx, n, nq, eng.batch_config(2048, 64, 4, order, threads))
+    plan.set_queries(queries)
+    assert plan.stats()["groups"] == -(-sum(shape) // slots)
+    stride = ((n + 31) // 32 + 3) // 4 * 4
+    out = eng.DeviceBuffer(4 * stride * nq)
+    counts = eng.DeviceBuffer(8 * nq)
+    out.fill(0x5A)
+    plan.launch(out, stride, counts)
+    check_results(eng, cols, queries, out, counts, stride)
+
+
 def test_mix_digest_tracks_the_query_distribution(eng):
     n = 1 << 18
     cols, dcols, idx, sc = make_table(eng, n, seed=23)
