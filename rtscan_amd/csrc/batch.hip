@@ -464,7 +464,16 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int64_t it_next = (int64_t)lds.next;
 
     // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
-    //    its own words from here on.
+    //    its own words from here on.  In the last tile, the thread's words [w, w + kPer)
+    //    hold nvalid words of the bitmap, the last of them the bitmap's last word when
+    //    holds_last (32-bit values, worked out once per item).
+    int nvalid = kPer;
+    bool holds_last = false;
+    if (!interior) {
+      const int64_t left = words - w;
+      nvalid = left <= 0 ? 0 : left >= kPer ? kPer : (int)left;
+      holds_last = nvalid > 0 && left <= kPer;
+    }
     for (int qi = 0; qi < nq; qi++) {
       uint32_t acc[kPer];
       const uint32_t fill = lds.qempty[qi] ? 0u : ~0u;
@@ -487,23 +496,23 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 #pragma unroll
         for (int k = 0; k < kPer; k++) pc += __popc(acc[k]);
         store_words<kPer>(lds.qout[qi] + w, acc);
-      } else if (w < words) {
+      } else if (nvalid > 0) {
         BITS *o = lds.qout[qi] + w;
 #pragma unroll
         for (int k = 0; k < kPer; k++) {
-          if (w + k == words - 1) acc[k] &= last_mask;
-          if (w + k >= words) acc[k] = 0;
+          if (k >= nvalid) acc[k] = 0;
+          else if (k == nvalid - 1 && holds_last) acc[k] &= last_mask;
           pc += __popc(acc[k]);
         }
-        if (w + kPer <= words) {
+        if (nvalid == kPer) {
           store_words<kPer>(o, acc);
         } else {
 #pragma unroll
           for (int k = 0; k < kPer; k++)
-            if (w + k < words) gmem(o)[k] = acc[k];
+            if (k < nvalid) gmem(o)[k] = acc[k];
         }
       }
-      pc = wave_sum(pc);
+      pc = wave_total(pc);  // DPP steps: no LDS round trips inside the AND phase
       if ((tid & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
     }
     for (int s = 0; s < nslots; s++) {

@@ -33,13 +33,6 @@ __device__ __forceinline__ uint32_t or_lanes8(uint32_t x) {
   return x;
 }
 
-// Sum over the 64 lanes of a wave (result valid in every lane).
-__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
-#pragma unroll
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
-  return v;
-}
-
 // Sum over the 64 lanes of a wave by DPP (rows of 16 by quad permutes and rotates, then
 // the row broadcasts), the total read from lane 63: a scalar.
 __device__ __forceinline__ uint32_t wave_total(uint32_t v) {
