@@ -91,16 +91,21 @@ struct BatchGroup {
   BatchQuery query[kBatchMaxGroup];
 };
 
+// A query of the current item as the AND phase reads it: one 16-byte LDS read.
+struct alignas(16) BatchQueryLds {
+  BITS *out;
+  uint32_t slots;  // first slot | one past its last slot << 8
+  uint32_t empty;
+};
+
 // LDS of one workgroup: one toggle array per slot and the current item's query descriptors.
 template <int TW, int NT>
 struct BatchLds {
   static constexpr int kSlots = BatchShape<NT>::kLdsWords / TW;
   static constexpr int kLists = 2 * kSlots;
   alignas(16) uint32_t tog[kSlots][TW];
-  BITS *qout[kBatchMaxGroup];
-  uint32_t qempty[kBatchMaxGroup];
+  BatchQueryLds q[kBatchMaxGroup];
   uint32_t qid[kBatchMaxGroup];
-  uint32_t qslot[kBatchMaxGroup + 1];
   uint32_t qcount[kBatchMaxGroup];
   unsigned long long next;  // the workgroup's next work item
 };
@@ -388,9 +393,12 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int qi = tid - 2 * kWave;
     const bool qload = tid >= 2 * kWave && qi < nq;
     BatchQuery Q{};
-    if (qload) Q = g->query[qi];
-    uint32_t qs = 0;
-    if (tid <= nq) qs = g->qslot[tid];
+    uint32_t qs0 = 0, qs1 = 0;
+    if (qload) {
+      Q = g->query[qi];
+      qs0 = g->qslot[qi];
+      qs1 = g->qslot[qi + 1];
+    }
 
     const int64_t w = t * TW + lw;
     const bool interior = (t + 1) * TW < words;  // the last bitmap word lies in a later tile
@@ -451,11 +459,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     for (int s = 0; s < nslots; s++)
       if (g->slot[s].col) batch_direct<TW, NT>(g->slot[s], t, n, lds.tog[s]);
     if (qload) {
-      lds.qout[qi] = Q.out ? Q.out : out + (int64_t)Q.id * out_stride;
-      lds.qempty[qi] = Q.empty;
+      lds.q[qi] = BatchQueryLds{Q.out ? Q.out : out + (int64_t)Q.id * out_stride, qs0 | qs1 << 8, (uint32_t)Q.empty};
       lds.qid[qi] = (uint32_t)Q.id;
     }
-    if (tid <= nq) lds.qslot[tid] = qs;
     // the next item's index, here rather than after the AND phase: waiting for the atomic
     // there would also wait for all of this item's bitmap stores (one counter covers both)
     if (tid == 0) lds.next = nblk_x + nxt;
@@ -464,9 +470,10 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const int64_t it_next = (int64_t)lds.next;
 
     // 3. each query: AND of its slots, stored with its popcount.  Every thread touches only
-    //    its own words from here on.  In the last tile, the thread's words [w, w + kPer)
-    //    hold nvalid words of the bitmap, the last of them the bitmap's last word when
-    //    holds_last (32-bit values, worked out once per item).
+    //    its own words from here on (and clears them for the next item).  In the last
+    //    tile, the thread's words [w, w + kPer) hold nvalid words of the bitmap, the last
+    //    of them the bitmap's last word when holds_last (32-bit values, worked out once
+    //    per item).
     int nvalid = kPer;
     bool holds_last = false;
     if (!interior) {
@@ -474,30 +481,55 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       nvalid = left <= 0 ? 0 : left >= kPer ? kPer : (int)left;
       holds_last = nvalid > 0 && left <= kPer;
     }
+    // this thread's words of slot s, read and cleared for the next item (each slot belongs
+    // to exactly one query, so each is read once)
+    auto take_slot = [&](uint32_t s, uint32_t(&v)[kPer]) {
+      if constexpr (TL::kUnit == 2) {
+#pragma unroll
+        for (int k = 0; k < TL::kUnits; k++) {
+          uint2 *u = reinterpret_cast<uint2 *>(&lds.tog[s][0]) + k * kBatchThreads + tid;
+          const uint2 x = *u;
+          *u = make_uint2(0, 0);
+          v[2 * k] = x.x;
+          v[2 * k + 1] = x.y;
+        }
+      } else {
+        v[0] = lds.tog[s][tid];
+        lds.tog[s][tid] = 0;
+      }
+    };
     for (int qi = 0; qi < nq; qi++) {
+      const BatchQueryLds D = lds.q[qi];
+      const uint32_t slots = uniform_u32(D.slots);
+      const uint32_t s0 = slots & 0xFF, s1 = slots >> 8;
       uint32_t acc[kPer];
-      const uint32_t fill = lds.qempty[qi] ? 0u : ~0u;
+      const uint32_t fill = D.empty ? 0u : ~0u;
 #pragma unroll
       for (int k = 0; k < kPer; k++) acc[k] = fill;
-      for (uint32_t s = lds.qslot[qi]; s < lds.qslot[qi + 1]; s++) {
-        if constexpr (TL::kUnit == 2) {
+      // a query has at most kBatchMaxCols slots, taken two at a time: the words of both
+      // are read before either is used
 #pragma unroll
-          for (int k = 0; k < TL::kUnits; k++) {
-            const uint2 v = reinterpret_cast<const uint2 *>(&lds.tog[s][0])[k * kBatchThreads + tid];
-            acc[2 * k] &= v.x;
-            acc[2 * k + 1] &= v.y;
+      for (int j = 0; j < kBatchMaxCols; j += 2) {
+        if (s0 + j < s1) {
+          uint32_t a[kPer], b[kPer];
+          take_slot(s0 + j, a);
+          if (s0 + j + 1 < s1) {
+            take_slot(s0 + j + 1, b);
+          } else {
+#pragma unroll
+            for (int k = 0; k < kPer; k++) b[k] = ~0u;
           }
-        } else {
-          acc[0] &= lds.tog[s][tid];
+#pragma unroll
+          for (int k = 0; k < kPer; k++) acc[k] &= a[k] & b[k];
         }
       }
       uint32_t pc = 0;
       if (interior) {  // every word of the tile is a full word of the bitmap
 #pragma unroll
         for (int k = 0; k < kPer; k++) pc += __popc(acc[k]);
-        store_words<kPer>(lds.qout[qi] + w, acc);
+        store_words<kPer>(D.out + w, acc);
       } else if (nvalid > 0) {
-        BITS *o = lds.qout[qi] + w;
+        BITS *o = D.out + w;
 #pragma unroll
         for (int k = 0; k < kPer; k++) {
           if (k >= nvalid) acc[k] = 0;
@@ -514,15 +546,6 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       }
       pc = wave_total(pc);  // DPP steps: no LDS round trips inside the AND phase
       if ((tid & (kWave - 1)) == 0 && pc) atomicAdd(&lds.qcount[qi], pc);
-    }
-    for (int s = 0; s < nslots; s++) {
-      if constexpr (TL::kUnit == 2) {
-#pragma unroll
-        for (int k = 0; k < TL::kUnits; k++)
-          reinterpret_cast<uint2 *>(&lds.tog[s][0])[k * kBatchThreads + tid] = make_uint2(0, 0);
-      } else {
-        lds.tog[s][tid] = 0;
-      }
     }
     // past the barrier, faster waves may already store the next item's query descriptors:
     // read qid before it
