@@ -449,10 +449,14 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         uint32_t *tg = lds.tog[lslot[i]];
 #pragma unroll
         for (int u = 0; u < kListUnroll; u++) {
-          const uint32_t e = off + u * kWave + lane;
-          // rows of the slice lie in [row0, row0 + TW * 32): the word index is < TW
-          if (e < llen[i] && ent[i][u].y - thr <= span)
-            atomicXor(&tg[tog_index<TW, NT>((ent[i][u].x - row0) >> 5)], 0x80000000u >> (ent[i][u].x & 31));
+          // a block of kWave entries past the slice's end is skipped by a wave-uniform
+          // branch (slices are a few hundred entries, so most rounds leave blocks unused)
+          if (off + u * kWave < llen[i]) {
+            const uint32_t e = off + u * kWave + lane;
+            // rows of the slice lie in [row0, row0 + TW * 32): the word index is < TW
+            if (e < llen[i] && ent[i][u].y - thr <= span)
+              atomicXor(&tg[tog_index<TW, NT>((ent[i][u].x - row0) >> 5)], 0x80000000u >> (ent[i][u].x & 31));
+          }
         }
       }
     }
