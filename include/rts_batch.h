@@ -100,6 +100,11 @@ void rts_batch_plan_stats(const rts_batch_plan *plan, int *bindex_cols, int *dir
 /* RTS_EVAL_AUTO query-columns of the current queries that the cost model sent to the
  * direct form although their column has an index (-1 for a NULL plan). */
 int rts_batch_plan_auto_direct(const rts_batch_plan *plan);
+/* BinDex query-columns of the current queries in the list form: a range over one or two
+ * bins whose refine lists hold all their rows, read from those lists alone with no filter
+ * vector (-1 for a NULL plan).  No counterpart in the reference, whose lists leave out
+ * every bin's start value. */
+int rts_batch_plan_list_cols(const rts_batch_plan *plan);
 /* 64-bit digest of the current query mix: mean columns per query (in quarters) and the
  * shares of query-columns per evaluation mode and per range width (empty, 1-8, 9-32 or more
  * of the column's 128 areas), in sixteenths.  Batches drawn from one distribution get the

@@ -120,6 +120,7 @@ SIGNATURES = {
     "rts_batch_plan_free": (None, [_vp]),
     "rts_batch_plan_groups": (_int, [_vp, _vp]),
     "rts_batch_plan_auto_direct": (_int, [_vp]),
+    "rts_batch_plan_list_cols": (_int, [_vp]),
     "rts_batch_plan_mix_digest": (ctypes.c_ulonglong, [_vp]),
     "rts_batch_plan_add_indexes": (_err, [_vp, _vp, ctypes.POINTER(_int)]),
     "rts_batch_autotune": (_err, [_vp, _vp, ctypes.c_int64, _vp, _int, _int, ctypes.c_char_p, _vp,
@@ -703,6 +704,7 @@ class BatchPlan:
                                    ctypes.byref(nt))
         return {"bindex_cols": b.value, "direct_cols": d.value, "narrow_cols": nr.value,
                 "tiles": nt.value, "auto_direct_cols": lib().rts_batch_plan_auto_direct(self.h),
+                "list_cols": lib().rts_batch_plan_list_cols(self.h),
                 "groups": lib().rts_batch_plan_groups(self.h, None)}
 
     def config(self):

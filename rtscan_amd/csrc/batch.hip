@@ -603,6 +603,7 @@ struct PlannedCol {
   int col = 0;
   bool direct = false;
   bool narrow = false;  // both bounds in one bin: one list pass over [lo_v, hi_v]
+  int list_b0 = -1, list_b1 = -1;  // list form: bins list_b0..list_b1, rows in [lo_v, hi_v]
   CODE lo_v = 0, hi_v = 0;
   CODE dlo = 0, dspan = 0;
   PlannedBound lo, hi;
@@ -622,6 +623,23 @@ static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   b.list = s.list;
   b.thr = s.thr;
   return b;
+}
+
+// List form of a BinDex query-column [lo, hi] (bounds sl, sh): the bins it covers are
+// sl's own bin (only when sl refines: its rows above lo, which its list always holds) and
+// the bins after it up to sh's bin; when these are at most two and every one but sl's own
+// is complete (its list holds all its rows), the column's rows are the rows in [lo, hi]
+// of those bins' lists, with no filter vector.  Two such lists are the very lists the
+// filter-vector form refines, so the list form reads strictly less.
+static bool list_form(const rts_bindex_dev *d, const BoundSpec &sl, const BoundSpec &sh, int *b0, int *b1) {
+  if ((int)d->complete.size() != d->bins) return false;
+  const int first = sl.list >= 0 ? sl.fv : sl.fv + 1, last = sh.fv;
+  if (first < 0 || last < first || last - first > 1 || last >= d->bins) return false;
+  for (int k = sl.fv + 1; k <= last; k++)
+    if (k >= first && !d->complete[k]) return false;
+  *b0 = first;
+  *b1 = last;
+  return true;
 }
 
 // Planner cost model of one query-column, in bytes per row of the column.  BinDex form:
@@ -678,6 +696,7 @@ struct rts_batch_plan {
   int graph_nq = -1;
   int64_t graph_stride = 0;
   int stats_bindex_cols = 0, stats_direct_cols = 0, stats_narrow_cols = 0, stats_auto_direct = 0;
+  int stats_list_cols = 0;
   std::vector<rts_query_spec> specs;  // the current queries, re-planned on a new tiling
   std::vector<BITS *> outs;           // their per-query outputs, if given
   bool has_outs = false;
@@ -925,7 +944,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   p->stats_auto_direct = 0;
   std::vector<PlannedCol> pc((size_t)nq * kBatchMaxCols);
   std::vector<int> qcols(nq, 0), qempty(nq, 0);
-  int st_bindex = 0, st_direct = 0, st_narrow = 0;
+  int st_bindex = 0, st_direct = 0, st_narrow = 0, st_list = 0;
   for (int q = 0; q < nq; q++) {
     const rts_query_spec &in = queries[q];
     if (in.ncols < 1 || in.ncols > kBatchMaxCols)
@@ -947,7 +966,9 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       if (!direct) {
         sl = lo == 0 ? BoundSpec{-1, -1, 0} : bound_spec(b->dev, lo, false);
         sh = bound_spec(b->dev, hi + 1u, hi == 0xFFFFFFFFu);
-        if (in.mode[c] == RTS_EVAL_AUTO && aligned && direct_cheaper(b->dev, sl, sh, p->cfg.direct_weight)) {
+        int b0, b1;
+        if (in.mode[c] == RTS_EVAL_AUTO && aligned && !list_form(b->dev, sl, sh, &b0, &b1) &&
+            direct_cheaper(b->dev, sl, sh, p->cfg.direct_weight)) {
           direct = true;
           p->stats_auto_direct++;
         }
@@ -966,6 +987,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
       P.hi = plan_bound(d, sh);
       // Both bounds in one bin: the result is the bin's list rows in [lo, hi] alone,
       // (1 ^ t_hi) & ~(1 ^ t_lo) = t_lo & ~t_hi, so no filter vector is read.
+      int b0, b1;
       if (sl.fv == sh.fv && sl.list >= 0 && sl.list == sh.list) {
         P.lo.base = P.hi.base = kBaseOnes;
         P.lo.fv = P.hi.fv = nullptr;
@@ -973,6 +995,14 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
         P.lo_v = lo;
         P.hi_v = hi;
         st_narrow++;
+      } else if (list_form(d, sl, sh, &b0, &b1)) {
+        P.lo.base = P.hi.base = kBaseZero;  // (0 & ~0) ^ T = T: the lists' rows alone
+        P.lo.fv = P.hi.fv = nullptr;
+        P.list_b0 = b0;
+        P.list_b1 = b1;
+        P.lo_v = lo;
+        P.hi_v = hi;
+        st_list++;
       }
       st_bindex++;
     }
@@ -1028,6 +1058,15 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
           L.thr = P.lo_v;
           L.span = P.hi_v - P.lo_v;
           L.slot = (uint32_t)g.nslots;
+        } else if (P.list_b0 >= 0) {  // the list form: its bins' rows with lo <= x <= hi
+          for (int k = P.list_b0; k <= P.list_b1; k++) {
+            BatchList &L = g.list[g.nlists++];
+            L.dir = p->dir_base[P.col] + (size_t)k * per;
+            L.bent = d->bent;
+            L.thr = P.lo_v;
+            L.span = P.hi_v - P.lo_v;
+            L.slot = (uint32_t)g.nslots;
+          }
         } else {
           for (const PlannedBound *B : {&P.lo, &P.hi}) {
             if (B->list < 0) continue;
@@ -1090,6 +1129,7 @@ hipError_t rts_batch_plan_set_queries_out(rts_batch_plan *p, const rts_query_spe
   p->stats_bindex_cols = st_bindex;
   p->stats_direct_cols = st_direct;
   p->stats_narrow_cols = st_narrow;
+  p->stats_list_cols = st_list;
   p->nq = nq;
   p->specs.assign(queries, queries + nq);
   p->h_groups.swap(groups);
@@ -1153,6 +1193,8 @@ void rts_batch_plan_stats(const rts_batch_plan *p, int *bindex_cols, int *direct
 }
 
 int rts_batch_plan_auto_direct(const rts_batch_plan *p) { return p ? p->stats_auto_direct : -1; }
+
+int rts_batch_plan_list_cols(const rts_batch_plan *p) { return p ? p->stats_list_cols : -1; }
 
 unsigned long long rts_batch_plan_mix_digest(const rts_batch_plan *p) {
   if (!p) return 0;

@@ -71,14 +71,19 @@ __global__ void k_rank_samples(const CODE *sorted, int64_t m, CODE *out) {
 }
 
 // One pass over the column: all bins-1 filter vectors (fv[k] bit i = x_i < estart[k+1],
-// set_fv_val_less, bindex.cpp:553-560) and each row's refine key (its bin if
-// estart[bin] < x, else `bins` = no refine).  One row per lane; a 64-row chunk gives two
-// MSB-first words per filter vector from one ballot.
+// set_fv_val_less, bindex.cpp:553-560) and each row's list key (its bin if estart[bin] <
+// x, or x = estart[bin] with light[bin] set, else `bins` = in no list).  One row per lane;
+// a 64-row chunk gives two MSB-first words per filter vector from one ballot.
 __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n,
-                                                      const CODE *starts_g, int bins, BITS *fvs,
-                                                      int64_t fv_stride, uint16_t *keys) {
+                                                      const CODE *starts_g, const uint8_t *light_g,
+                                                      int bins, BITS *fvs, int64_t fv_stride,
+                                                      uint16_t *keys) {
   __shared__ CODE starts[kMaxBins];
-  for (int i = threadIdx.x; i < bins; i += blockDim.x) starts[i] = starts_g[i];
+  __shared__ uint8_t light[kMaxBins];
+  for (int i = threadIdx.x; i < bins; i += blockDim.x) {
+    starts[i] = starts_g[i];
+    light[i] = light_g[i];
+  }
   __syncthreads();
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
@@ -98,7 +103,7 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
         hi = mid;
     }
     const int t = lo - 1;
-    if (valid) keys[r] = starts[t] < x ? (uint16_t)t : (uint16_t)bins;
+    if (valid) keys[r] = starts[t] < x || light[t] ? (uint16_t)t : (uint16_t)bins;
     const int64_t w0 = ch * 2;
     const bool w1_valid = (w0 + 1) * 32 < n;
     for (int k = 0; k < bins - 1; k++) {
@@ -185,6 +190,60 @@ static unsigned blocks_for(int64_t items, int per = kBlock) {
       goto fail;                        \
     }                                   \
   } while (0)
+
+// rk[2k] = first sorted slot with value >= estarts[k], rk[2k + 1] = first with value >
+// estarts[k] (binary searches over the sorted column, one thread per bin start).
+__global__ void k_start_ranks(const CODE *sorted, int64_t n, const CODE *estarts, int bins, int64_t *rk) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= bins) return;
+  const CODE v = estarts[k];
+  int64_t lo = 0, hi = n;  // first slot with sorted >= v
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] < v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rk[2 * k] = lo;
+  hi = n;  // first slot with sorted > v
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if (sorted[mid] <= v)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  rk[2 * k + 1] = lo;
+}
+
+hipError_t start_ranks(const rts_bindex_dev *d, std::vector<int64_t> &rk, hipStream_t s) {
+  const int bins = d->bins;
+  CODE *est = nullptr;
+  int64_t *rk_d = nullptr;
+  rk.assign(2 * (size_t)bins, 0);
+  hipError_t e = hipMalloc((void **)&est, bins * sizeof(CODE));
+  if (e == hipSuccess) e = hipMalloc((void **)&rk_d, 2 * (size_t)bins * sizeof(int64_t));
+  if (e == hipSuccess) e = hipMemcpyAsync(est, d->estarts.data(), bins * sizeof(CODE), hipMemcpyHostToDevice, s);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(k_start_ranks, dim3((bins + kBlock - 1) / kBlock), dim3(kBlock), 0, s, d->sorted_val, d->n,
+                       est, bins, rk_d);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(rk.data(), rk_d, 2 * (size_t)bins * sizeof(int64_t), hipMemcpyDeviceToHost, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(est);
+  (void)hipFree(rk_d);
+  return e;
+}
+
+void mark_complete_bins(rts_bindex_dev *d, const std::vector<int64_t> &rk) {
+  d->complete.assign(d->bins, 0);
+  for (int k = 0; k < d->bins; k++) {
+    const int64_t rows = (k + 1 < d->bins ? rk[2 * (k + 1)] : d->n) - rk[2 * k];
+    d->complete[k] = (int64_t)(d->boff_host[k + 1] - d->boff_host[k]) == rows;
+  }
+}
 
 hipError_t rank_samples(rts_bindex_dev *d, hipStream_t s) {
   const int64_t m = (d->n + kRankSample - 1) / kRankSample;
@@ -309,6 +368,9 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   int64_t idx_h[kAreas];
   uint32_t nb = 0;
   int bins = 0, key_bits = 0;
+  std::vector<int64_t> rk;
+  uint8_t light_h[kMaxBins];
+  uint8_t *light_d = nullptr;
 
   if (!bx || !col || n < kAreas || n > 0x7FFFFFFFll || (sub != 1 && sub != 2 && sub != 4 && sub != 8))
     return hipErrorInvalidValue;
@@ -351,11 +413,17 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   RTS_TRY(hipMemcpyAsync(idx_h, idx_d, sizeof(idx_h), hipMemcpyDeviceToHost, s));
   RTS_TRY(hipMemcpyAsync(&bx->data_max, d->sorted_val + n - 1, sizeof(CODE), hipMemcpyDeviceToHost, s));
 
-  // 3. filter vectors + refine keys
+  // 3. filter vectors + list keys; a bin start of at most kLightStartRows rows goes into
+  //    its bin's list
+  RTS_TRY(hipStreamSynchronize(s));  // d->estarts
+  RTS_TRY(start_ranks(d, rk, s));
+  for (int k = 0; k < bins; k++) light_h[k] = rk[2 * k + 1] - rk[2 * k] <= kLightStartRows;
+  RTS_TRY(hipMalloc((void **)&light_d, bins));
+  RTS_TRY(hipMemcpyAsync(light_d, light_h, bins, hipMemcpyHostToDevice, s));
   RTS_TRY(hipMalloc((void **)&d->fvs, (size_t)(bins - 1) * d->fv_stride * sizeof(BITS)));
   RTS_TRY(hipMalloc((void **)&keys, n * sizeof(uint16_t)));
   hipLaunchKernelGGL(k_build_fvs, dim3(blocks_for((n + 63) / 64, kBlock / kWave)), dim3(kBlock), 0,
-                     s, col, n, estarts_d, bins, d->fvs, d->fv_stride, keys);
+                     s, col, n, estarts_d, light_d, bins, d->fvs, d->fv_stride, keys);
   RTS_TRY(hipGetLastError());
 
   // 4. B lists: stable sort of (key, row) -> rows grouped by bin in row order
@@ -380,6 +448,7 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   RTS_TRY(hipStreamSynchronize(s));
   nb = d->boff_host[bins];
   d->nb = nb;
+  mark_complete_bins(d, rk);
   RTS_TRY(hipMalloc((void **)&d->bent, std::max<uint32_t>(nb, 1) * sizeof(uint2)));
   if (nb) {
     hipLaunchKernelGGL(k_gather_entries, dim3(blocks_for(nb)), dim3(kBlock), 0, s, col, bpos_in,
@@ -433,6 +502,7 @@ fail:
   (void)hipFree(tmp);
   (void)hipFree(starts_d);
   (void)hipFree(estarts_d);
+  (void)hipFree(light_d);
   (void)hipFree(idx_d);
   (void)hipFree(bf_slot_d);
   (void)hipFree(bl_slot_d);
@@ -700,6 +770,13 @@ hipError_t bindex_load(BinDex *bx, const char *path, const CODE *col) {
   if (e == hipSuccess) e = launch_tile_directory(d, d->ntiles, kTileRows, d->bdir, ref_stream());
   if (e == hipSuccess) e = hipStreamSynchronize(ref_stream());
   if (e == hipSuccess) e = rank_samples(d, ref_stream());
+  if (e == hipSuccess) {
+    // which bins' lists hold every row of the bin (a snapshot of an index built without
+    // the start rows in its lists has none)
+    std::vector<int64_t> rk;
+    e = start_ranks(d, rk, ref_stream());
+    if (e == hipSuccess) mark_complete_bins(d, rk);
+  }
   if (e == hipSuccess) {
     memset(bx, 0, sizeof(*bx));
     bx->data_max = h.data_max;

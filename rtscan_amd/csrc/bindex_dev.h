@@ -9,9 +9,12 @@
 //                             fv_stride words apart (stride padded to 64 words); the
 //                             reference's filter vector of area a is fv[(a + 1) * sub - 1]
 //   B lists                   for every bin k (value range [estarts[k], estarts[k+1])), the
-//                             rows with estarts[k] < x < estarts[k+1] in row order, one
-//                             8-byte entry per row: bent[e] = (row, value); bin k's entries
-//                             are [boff[k], boff[k+1])
+//                             rows with estarts[k] < x < estarts[k+1] in row order, and the
+//                             rows with x = estarts[k] too when there are at most
+//                             kLightStartRows of them; one 8-byte entry per row:
+//                             bent[e] = (row, value); bin k's entries are [boff[k],
+//                             boff[k+1]); complete[k] says whether they are all of bin k's
+//                             rows
 //   bdir                      tile directory: bin k's first entry with row >= t * kTileRows
 //                             is bdir[k * (ntiles + 1) + t]
 //   rank_samples              every kRankSample-th sorted value (host): rank_samples[i] =
@@ -19,9 +22,12 @@
 // A bound v of a query (the set L(v) = {x < v}) is then one filter vector plus the rows of
 // one bin's B list with x >= v: L(v) = fv[t(v)] minus {B_t(v) rows with x >= v}, or
 // fv[t(v) - 1] exactly when v is a bin start.  Rows equal to a bin start never need a
-// refine, so a heavily duplicated value (Zipf) costs nothing at query time.  More bins
-// mean shorter refine lists per bound (2n/bins entries at most) for (bins - 1) * n / 8
-// bytes of filter vectors.
+// refine (a refine threshold lies above its bin's start), so a heavily duplicated value
+// (Zipf) is left out of its bin's list and costs nothing at query time; a start value of
+// few rows is kept in the list, which then holds the whole bin, so that a range covering
+// only a few complete bins can be answered from their lists alone.  More bins mean
+// shorter refine lists per bound (2n/bins entries at most) for (bins - 1) * n / 8 bytes of
+// filter vectors.
 #pragma once
 
 #include <stdint.h>
@@ -42,6 +48,7 @@ constexpr int kTileWords = 1024;              // bitmap words per query-kernel t
 constexpr int kTileRows = kTileWords * 32;    // 32768 rows
 constexpr int kFvStrideAlign = 64;            // words
 constexpr int64_t kRankSample = 256;          // sorted slots per rank sample
+constexpr int64_t kLightStartRows = 64;       // a bin start of at most this many rows is listed
 
 }  // namespace rts
 
@@ -67,6 +74,7 @@ struct rts_bindex_dev {
   std::vector<CODE> estarts;         // bins
   std::vector<uint32_t> boff_host;   // bins + 1
   std::vector<CODE> rank_samples;    // ceil(n / kRankSample)
+  std::vector<uint8_t> complete;     // bins: bin k's B list holds every row of bin k
   std::vector<Area> area_store;      // BinDex::areas
   std::vector<Area> area_gpu_store;  // BinDex::areasInGPU
   std::vector<CODE> block_first, block_last;
@@ -115,6 +123,14 @@ hipError_t launch_tile_directory(const rts_bindex_dev *d, int64_t ntiles, int64_
 
 // d->rank_samples (host) from the device's sorted values, on s (synchronous).
 hipError_t rank_samples(rts_bindex_dev *d, hipStream_t s);
+
+// Ranks of the bin starts in the device's sorted values (synchronous): rk[2k] = #{x <
+// estarts[k]}, rk[2k + 1] = #{x <= estarts[k]}.
+hipError_t start_ranks(const rts_bindex_dev *d, std::vector<int64_t> &rk, hipStream_t s);
+
+// d->complete from the start ranks and d->boff_host: bin k's list is complete when it has
+// as many entries as bin k has rows.
+void mark_complete_bins(rts_bindex_dev *d, const std::vector<int64_t> &rk);
 
 // Sorted-slot interval [*L, *R] (fewer than kRankSample slots apart) that holds rank(v) =
 // #{x < v}, from the host's samples (v = 2^32 when infinite: rank n).  With c samples below

@@ -11,6 +11,7 @@ import oracle as O
 import snapshot_format as S
 from _util import MAXU, interval_mask, pack_rows
 from test_gpu_batch import check_results, make_table, random_queries
+from test_gpu_bindex import engine_bin_starts
 
 pytestmark = pytest.mark.gpu
 
@@ -223,6 +224,55 @@ def test_groups_are_packed_to_the_fewest_work_items(eng, threads, order):
     out.fill(0x5A)
     plan.launch(out, stride, counts)
     check_results(eng, cols, queries, out, counts, stride)
+
+
+@pytest.mark.parametrize("name", ["uniform", "zipf15", "small_range", "split_heavy"])
+def test_list_form_ranges_at_bin_edges_are_exact(eng, tmp_path, name):
+    """A range over one or two bins whose refine lists hold every row of the bin (a bin
+    start of at most 64 rows is listed with its bin) is read from those lists alone, with
+    no filter vector.  Ranges at and around the engine bin starts of a 256-bin index: one
+    bin, two bins, bounds one inside a bin, three bins (filter vectors again), and the
+    same through a snapshot of the index.  A heavy bin start (small_range: every value is
+    heavy) leaves its bin incomplete, so only a range from inside one bin to that bin's
+    end keeps the list form there.  Every bitmap and count equals numpy's."""
+    n = 1_000_003
+    x = datasets(eng, n)[name]
+    d = eng.DeviceBuffer.from_numpy(x)
+    ix = eng.BinDexIndex(d, n, bins=256)
+    sv = np.sort(x)
+    st = engine_bin_starts(ix, sv, 256) + [MAXU + 1]
+    qs = []
+    for b in (0, 1, 2, 17, 63, 100, 127, 128, 200, 253, 254, 255):
+        lo, nxt = st[b], st[min(b + 1, 256)]
+        for hi_bin in (b + 1, b + 2, b + 3):
+            hi = st[min(hi_bin, 256)] - 1
+            if lo > MAXU or hi < lo:
+                continue
+            qs.append([(0, lo, min(hi, MAXU), 0)])
+            if lo + 1 <= min(hi, MAXU):
+                qs.append([(0, lo + 1, min(hi, MAXU), 0)])
+            if nxt <= MAXU and lo < nxt - 1:
+                qs.append([(0, lo + 1, nxt, 0)])
+    plan = eng.BatchPlan([d], [ix], n, len(qs), eng.batch_config(2048, 64, 4, 1, 256))
+    plan.set_queries(qs)
+    stride = ((n + 31) // 32 + 3) // 4 * 4
+    out = eng.DeviceBuffer(4 * stride * len(qs))
+    counts = eng.DeviceBuffer(8 * len(qs))
+    out.fill(0x3C)
+    plan.launch(out, stride, counts)
+    check_results(eng, [x], qs, out, counts, stride)
+    listed = plan.stats()["list_cols"]
+    if name == "uniform":
+        assert listed >= len(qs) // 3
+    path = str(tmp_path / "ix.rtsbdx")
+    ix.save(path)
+    loaded = eng.BinDexIndex(d, n, snapshot=path)
+    plan2 = eng.BatchPlan([d], [loaded], n, len(qs), eng.batch_config(2048, 64, 4, 1, 256))
+    plan2.set_queries(qs)
+    assert plan2.stats()["list_cols"] == listed
+    out.fill(0x3C)
+    plan2.launch(out, stride, counts)
+    check_results(eng, [x], qs, out, counts, stride)
 
 
 def test_mix_digest_tracks_the_query_distribution(eng):
