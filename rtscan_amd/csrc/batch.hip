@@ -406,8 +406,11 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     //    wave v of W owns lists v, v + W, ...; a list's slice base is uniform across the
     //    wave, each lane takes kListUnroll entries of each of its lists per round, and the
     //    first round's loads go out together with the filter-vector loads.
+    //    Two lists per wave take 6 entries per lane each (384 entries per list and round:
+    //    a 2048-word tile's slice of a 256-bin list is ~256), measured faster than 4, 5, 7
+    //    or 8 on the 4096-query batch.
     constexpr int kListsPerWave = kLists / kBatchWaves;
-    constexpr int kListUnroll = 16 / kListsPerWave;
+    constexpr int kListUnroll = kListsPerWave == 2 ? 6 : 16 / kListsPerWave;
     static_assert(kLists % kBatchWaves == 0, "lists per wave");
     const int lane = tid & (kWave - 1), wv = (int)uniform_u32(tid / kWave);
     const uint2 *lbase[kListsPerWave];
