@@ -3,8 +3,9 @@
 // A query is a conjunction of closed ranges over 1..4 columns of a table (all columns
 // have n rows).  Each query-column ("slot") is evaluated either from the column's BinDex
 // (two filter vectors and the tile slices of up to two bins' refine lists, as in
-// k_bindex_query) or directly from the column's values (the wave64 DPP word assembly of
-// k_scan_tiles).
+// k_bindex_query; or, for a range over one or two bins whose lists hold all their rows,
+// those lists alone) or directly from the column's values (the wave64 DPP word assembly
+// of k_scan_tiles).
 //
 // Work item = (row tile t, query group g): the workgroup writes the tile's output words of
 // every query of the group.  The items of one tile run on the same XCD one after another
@@ -20,9 +21,10 @@
 // entries and filter-vector words) and two barriers, so the launch is bound by latency:
 // each wave reads its own lists' records and directory words (wave-uniform addresses, no
 // exchange through LDS); refine entries are list-major (wave v of W owns lists v, v + W,
-// ...), 16 entries per lane in flight; the filter-vector words go out with the first refine
-// round and are XORed into the LDS toggles as they arrive; the query descriptors reach LDS
-// before the one barrier that precedes the AND phase.
+// ...), 12-16 entries per lane in flight; the filter-vector words go out with the first
+// refine round and are XORed into the LDS toggles as they arrive; the query descriptors
+// reach LDS before the barrier that precedes the AND phase, whose per-query work needs no
+// LDS round trip of its own beyond one descriptor read and two slots' words at a time.
 #include <string.h>
 
 #include <math.h>
