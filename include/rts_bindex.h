@@ -141,7 +141,8 @@ hipError_t rts_bindex_query(const BinDex *const *bindexes, const CODE *lo, const
 /* Host copies of the value-sorted column and its row positions (ties in row order). */
 hipError_t rts_bindex_export_sorted(const BinDex *bindex, CODE *host_sorted_val,
                                     int32_t *host_sorted_pos);
-/* Number of refine-list entries (rows strictly inside a bin), or -1. */
+/* Number of refine-list entries (rows strictly inside a bin, plus the rows of each bin start
+ * value that has at most 64 rows), or -1. */
 long long rts_bindex_refine_entries(const BinDex *bindex);
 /* Device bytes held by the index (excluding the column), or -1. */
 long long rts_bindex_device_bytes(const BinDex *bindex);
