@@ -630,23 +630,6 @@ static PlannedBound plan_bound(const rts_bindex_dev *d, const BoundSpec &s) {
   return b;
 }
 
-// List form of a BinDex query-column [lo, hi] (bounds sl, sh): the bins it covers are
-// sl's own bin (only when sl refines: its rows above lo, which its list always holds) and
-// the bins after it up to sh's bin; when these are at most two and every one but sl's own
-// is complete (its list holds all its rows), the column's rows are the rows in [lo, hi]
-// of those bins' lists, with no filter vector.  Two such lists are the very lists the
-// filter-vector form refines, so the list form reads strictly less.
-static bool list_form(const rts_bindex_dev *d, const BoundSpec &sl, const BoundSpec &sh, int *b0, int *b1) {
-  if ((int)d->complete.size() != d->bins) return false;
-  const int first = sl.list >= 0 ? sl.fv : sl.fv + 1, last = sh.fv;
-  if (first < 0 || last < first || last - first > 1 || last >= d->bins) return false;
-  for (int k = sl.fv + 1; k <= last; k++)
-    if (k >= first && !d->complete[k]) return false;
-  *b0 = first;
-  *b1 = last;
-  return true;
-}
-
 // Planner cost model of one query-column, in bytes per row of the column.  BinDex form:
 // the filter vectors it reads (4 bytes per 32 rows each) plus its refine lists' entries
 // (8 bytes each, spread over the column); a narrow column (both bounds in one bin) reads

@@ -116,6 +116,23 @@ inline BoundSpec bound_spec(const rts_bindex_dev *d, CODE v, bool infinite) {
   return BoundSpec{k, d->boff_host[k + 1] > d->boff_host[k] ? k : -1, v};
 }
 
+// List form of a BinDex query-column [lo, hi] (bounds sl, sh): the bins it covers are
+// sl's own bin (only when sl refines: its rows above lo, which its list always holds) and
+// the bins after it up to sh's bin; when these are at most two and every one but sl's own
+// is complete (its list holds all its rows), the column's rows are the rows in [lo, hi]
+// of those bins' lists, with no filter vector.  Two such lists are the very lists the
+// filter-vector form refines, so the list form reads strictly less.
+inline bool list_form(const rts_bindex_dev *d, const BoundSpec &sl, const BoundSpec &sh, int *b0, int *b1) {
+  if ((int)d->complete.size() != d->bins) return false;
+  const int first = sl.list >= 0 ? sl.fv : sl.fv + 1, last = sh.fv;
+  if (first < 0 || last < first || last - first > 1 || last >= d->bins) return false;
+  for (int k = sl.fv + 1; k <= last; k++)
+    if (k >= first && !d->complete[k]) return false;
+  *b0 = first;
+  *b1 = last;
+  return true;
+}
+
 // Tile directory of an index for tiles of `tile_rows` rows: bdir[k * (ntiles + 1) + t] =
 // first B-list entry of bin k with row >= t * tile_rows (asynchronous on s).
 hipError_t launch_tile_directory(const rts_bindex_dev *d, int64_t ntiles, int64_t tile_rows, uint32_t *bdir,

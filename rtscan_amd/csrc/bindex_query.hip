@@ -313,12 +313,27 @@ static hipError_t query_group(const BinDex *const *bxs, const CODE *lo, const CO
     const BoundSpec sh = bound_spec(d, hi[c] + 1u, hi[c] == 0xFFFFFFFFu);
     q.lo[c] = make_bound(d, sl);
     q.hi[c] = make_bound(d, sh);
+    int b0, b1;
     if (sl.list >= 0 && sl.list == sh.list) {
       // both bounds in one bin: (1 ^ 1) ^ t = t with t the bin's rows in [lo, hi], one pass
       // over the list and no filter vector
       q.lo[c].base = q.hi[c].base = kBaseOnes;
       q.lo[c].span = hi[c] - lo[c];
       q.hi[c].dir = nullptr;
+    } else if (list_form(d, sl, sh, &b0, &b1)) {
+      // one or two bins whose lists hold the column's rows: (0 & ~0) ^ t = t with t their
+      // rows in [lo, hi] (disjoint bins, so the XOR of the two lists is their union)
+      for (int j = 0; j < 2; j++) {
+        BoundDev &b = j ? q.hi[c] : q.lo[c];
+        b = BoundDev{};
+        b.base = kBaseZero;
+        if (j == 0 || b1 != b0) {
+          b.dir = d->bdir + (size_t)(j ? b1 : b0) * (d->ntiles + 1);
+          b.bent = d->bent;
+          b.thr = lo[c];
+          b.span = hi[c] - lo[c];
+        }
+      }
     }
   }
   RankArgs rank{};

@@ -232,7 +232,8 @@ def test_list_form_ranges_at_bin_edges_are_exact(eng, tmp_path, name):
     start of at most 64 rows is listed with its bin) is read from those lists alone, with
     no filter vector.  Ranges at and around the engine bin starts of a 256-bin index: one
     bin, two bins, bounds one inside a bin, three bins (filter vectors again), and the
-    same through a snapshot of the index.  A heavy bin start (small_range: every value is
+    same through a snapshot of the index and through the one-query kernel (alone and in a
+    two-column query).  A heavy bin start (small_range: every value is
     heavy) leaves its bin incomplete, so only a range from inside one bin to that bin's
     end keeps the list form there.  Every bitmap and count equals numpy's."""
     n = 1_000_003
@@ -264,6 +265,20 @@ def test_list_form_ranges_at_bin_edges_are_exact(eng, tmp_path, name):
     listed = plan.stats()["list_cols"]
     if name == "uniform":
         assert listed >= len(qs) // 3
+    # the one-query kernel takes the same list form: each range alone, counted, and as the
+    # second column of a two-column query with a full-range first column
+    W = (n + 31) // 32
+    one = eng.DeviceBuffer(4 * W)
+    c1 = eng.DeviceBuffer(8)
+    for q in qs[::3]:
+        (_c, lo, hi, _m) = q[0]
+        m = interval_mask(x, lo, hi)
+        eng.bindex_query([ix], [lo], [hi], one, c1)
+        assert np.array_equal(one.to_numpy(np.uint32, W), pack_rows(m)), (name, lo, hi)
+        assert int(c1.to_numpy(np.uint64, 1)[0]) == int(m.sum()), (name, lo, hi)
+        eng.bindex_query([ix, ix], [0, lo], [MAXU, hi], one, c1)
+        assert np.array_equal(one.to_numpy(np.uint32, W), pack_rows(m)), (name, lo, hi)
+        assert int(c1.to_numpy(np.uint64, 1)[0]) == int(m.sum()), (name, lo, hi)
     path = str(tmp_path / "ix.rtsbdx")
     ix.save(path)
     loaded = eng.BinDexIndex(d, n, snapshot=path)
