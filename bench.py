@@ -920,12 +920,14 @@ def serve_queries(rng, table_cols, nq, ncols_max):
 
 
 def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000_000_000,
-                seed=55, pool_gb=32, batch_config=None, bins=256):
+                seed=55, pool_gb=None, batch_config=None, bins=256):
     """Config 5: two resident tables, T1 = 2^26 rows x 2 uniform columns, T2 = n2 rows x 4
     columns (uniform, normal, Zipf 1.1, Zipf 1.5), a BinDex of `bins` bins per column (the
     count the autotuner picks for the config-4 batch); Poisson arrivals at
     each rate, every query on T1 or T2 with probability 1/2, shapes as config 4; admission by
-    the result pool's free HBM; continuous batching over 3 streams."""
+    the result pool's free HBM (a pool of up to 64 GB of what the tables leave free, 8 GB kept
+    back: 512 of the 125 MB bitmaps of the 1e9-row table); continuous batching of up to 512
+    requests over 3 streams."""
     t_setup = time.time()
     n1 = 1 << 26
     import oracle
@@ -943,7 +945,9 @@ def bench_serve(R, rates=(1000, 10000, 50000), seconds=(2.0, 1.0, 0.5), n2=1_000
         t2_ix.append(R.BinDexIndex(t2_d[-1], n2, bins=bins))
     t1_samples = [np.sort(c[rng.integers(0, n1, 1_000_000)]) for c in t1_cols]
     setup_s = time.time() - t_setup
-    cfg = R.server_config(max_batch=256, streams=3, pool_bytes=pool_gb << 30)
+    if pool_gb is None:
+        pool_gb = int(max(8, min(64, (R.device_free_bytes() >> 30) - 8)))
+    cfg = R.server_config(max_batch=512, streams=3, pool_bytes=pool_gb << 30)
     if batch_config is not None:
         cfg.batch = R.batch_config(*batch_config)
     srv = R.Server(cfg)
