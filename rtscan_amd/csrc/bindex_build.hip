@@ -72,8 +72,14 @@ __global__ void k_rank_samples(const CODE *sorted, int64_t m, CODE *out) {
 
 // One pass over the column: all bins-1 filter vectors (fv[k] bit i = x_i < estart[k+1],
 // set_fv_val_less, bindex.cpp:553-560) and each row's list key (its bin if estart[bin] <
-// x, or x = estart[bin] with light[bin] set, else `bins` = in no list).  One row per lane;
-// a 64-row chunk gives two MSB-first words per filter vector from one ballot.
+// x, or x = estart[bin] with light[bin] set, else `bins` = in no list).  A wave takes 2048
+// consecutive rows, which are 64 words of every filter vector: lane l finds the bins of
+// rows 64c + l (c = 0..31, binary search in LDS); then for each k, 32 ballots give the 64
+// words of fv[k] (the two MSB-first halves of ballot c are words 2c and 2c + 1, moved into
+// those lanes), stored by one coalesced instruction.  A row past n is in no filter vector;
+// the words past the last (up to the 64-word padded stride) are written as zeros.
+constexpr int kFvTileRows = 2048;
+
 __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n,
                                                       const CODE *starts_g, const uint8_t *light_g,
                                                       int bins, BITS *fvs, int64_t fv_stride,
@@ -88,28 +94,39 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
   const int lane = threadIdx.x & (kWave - 1);
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
-  const int64_t nchunks = (n + 63) / 64;
-  for (int64_t ch = wave; ch < nchunks; ch += nwaves) {
-    const int64_t r = ch * 64 + lane;
-    const bool valid = r < n;
-    const CODE x = valid ? col[r] : 0;
-    // t = number of k in [1, bins-1] with start[k] <= x (binary search in LDS)
-    int lo = 1, hi = bins;
-    while (lo < hi) {
-      const int mid = (lo + hi) >> 1;
-      if (starts[mid] <= x)
-        lo = mid + 1;
-      else
-        hi = mid;
+  const int64_t ntiles = (n + kFvTileRows - 1) / kFvTileRows;
+  constexpr int kChunks = kFvTileRows / kWave;  // 32 ballots of 64 rows
+  for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
+    const int64_t r0 = tile * kFvTileRows;
+    uint32_t bin[kChunks];
+#pragma unroll
+    for (int c = 0; c < kChunks; c++) {
+      const int64_t r = r0 + (int64_t)kWave * c + lane;
+      const bool valid = r < n;
+      const CODE x = valid ? col[r] : 0;
+      // t = number of k in [1, bins-1] with start[k] <= x
+      int lo = 1, hi = bins;
+      while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (starts[mid] <= x)
+          lo = mid + 1;
+        else
+          hi = mid;
+      }
+      const int t = lo - 1;
+      if (valid) keys[r] = starts[t] < x || light[t] ? (uint16_t)t : (uint16_t)bins;
+      bin[c] = valid ? (uint32_t)t : (uint32_t)bins;
     }
-    const int t = lo - 1;
-    if (valid) keys[r] = starts[t] < x || light[t] ? (uint16_t)t : (uint16_t)bins;
-    const int64_t w0 = ch * 2;
-    const bool w1_valid = (w0 + 1) * 32 < n;
+    BITS *dst = fvs + tile * (kFvTileRows / 32) + lane;
     for (int k = 0; k < bins - 1; k++) {
-      const uint64_t b = __ballot(valid && k >= t);
-      if (lane == 0) fvs[k * fv_stride + w0] = __brev((uint32_t)b);
-      if (lane == 1 && w1_valid) fvs[k * fv_stride + w0 + 1] = __brev((uint32_t)(b >> 32));
+      uint32_t word = 0;
+#pragma unroll
+      for (int c = 0; c < kChunks; c++) {
+        const uint64_t b = __ballot(bin[c] <= (uint32_t)k);
+        if (lane == 2 * c) word = __brev((uint32_t)b);
+        if (lane == 2 * c + 1) word = __brev((uint32_t)(b >> 32));
+      }
+      dst[k * fv_stride] = word;
     }
   }
 }
@@ -422,7 +439,7 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   RTS_TRY(hipMemcpyAsync(light_d, light_h, bins, hipMemcpyHostToDevice, s));
   RTS_TRY(hipMalloc((void **)&d->fvs, (size_t)(bins - 1) * d->fv_stride * sizeof(BITS)));
   RTS_TRY(hipMalloc((void **)&keys, n * sizeof(uint16_t)));
-  hipLaunchKernelGGL(k_build_fvs, dim3(blocks_for((n + 63) / 64, kBlock / kWave)), dim3(kBlock), 0,
+  hipLaunchKernelGGL(k_build_fvs, dim3(blocks_for((n + kFvTileRows - 1) / kFvTileRows, kBlock / kWave)), dim3(kBlock), 0,
                      s, col, n, estarts_d, light_d, bins, d->fvs, d->fv_stride, keys);
   RTS_TRY(hipGetLastError());
 
