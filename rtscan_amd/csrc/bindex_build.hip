@@ -73,13 +73,15 @@ __global__ void k_rank_samples(const CODE *sorted, int64_t m, CODE *out) {
 // One pass over the column: all bins-1 filter vectors (fv[k] bit i = x_i < estart[k+1],
 // set_fv_val_less, bindex.cpp:553-560) and each row's list key (its bin if estart[bin] <
 // x, or x = estart[bin] with light[bin] set, else `bins` = in no list).  A wave takes 2048
-// consecutive rows, which are 64 words of every filter vector: lane l finds the bins of
-// rows 64c + l (c = 0..31, binary search in LDS); then for each k, 32 ballots give the 64
-// words of fv[k] (the two MSB-first halves of ballot c are words 2c and 2c + 1, moved into
-// those lanes), stored by one coalesced instruction.  A row past n is in no filter vector;
-// the words past the last (up to the 64-word padded stride) are written as zeros.
+// consecutive rows, 64 words of every filter vector: lane l the 32 rows of word l.  The
+// lane finds its rows' bins t (binary search in LDS) and keeps them bit-sliced: NB words,
+// word b holding bit b of the 32 rows' t, MSB-first like the bitmaps.  Then for each k,
+// fv[k] = fv[k - 1] | {rows with t = k}, the equality taken over the NB slices (two
+// instructions a slice), and the 64 lanes store their words with one instruction.  A row
+// past n has t = bins, in no filter vector; the words past the last (up to the 64-word
+// padded stride) are written as zeros.  NB = bits of `bins` + 1 (t <= bins).
 constexpr int kFvTileRows = 2048;
-
+template <int NB>
 __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n,
                                                       const CODE *starts_g, const uint8_t *light_g,
                                                       int bins, BITS *fvs, int64_t fv_stride,
@@ -95,38 +97,67 @@ __global__ __launch_bounds__(kBlock) void k_build_fvs(const CODE *col, int64_t n
   const int64_t wave = (int64_t)blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave;
   const int64_t nwaves = (int64_t)gridDim.x * (kBlock / kWave);
   const int64_t ntiles = (n + kFvTileRows - 1) / kFvTileRows;
-  constexpr int kChunks = kFvTileRows / kWave;  // 32 ballots of 64 rows
+  const bool aligned = ((uintptr_t)col & 15) == 0;  // 16-byte loads of the column
   for (int64_t tile = wave; tile < ntiles; tile += nwaves) {
-    const int64_t r0 = tile * kFvTileRows;
-    uint32_t bin[kChunks];
+    const int64_t r0 = tile * kFvTileRows + 32 * lane;  // this lane's 32 rows
+    uint32_t slice[NB];
 #pragma unroll
-    for (int c = 0; c < kChunks; c++) {
-      const int64_t r = r0 + (int64_t)kWave * c + lane;
-      const bool valid = r < n;
-      const CODE x = valid ? col[r] : 0;
-      // t = number of k in [1, bins-1] with start[k] <= x
-      int lo = 1, hi = bins;
-      while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (starts[mid] <= x)
-          lo = mid + 1;
-        else
-          hi = mid;
+    for (int b = 0; b < NB; b++) slice[b] = 0;
+    const bool whole = r0 + 32 <= n;
+    uint32_t kw[16];  // the 32 rows' keys, two per word
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      uint4 v = make_uint4(0, 0, 0, 0);
+      if (whole && aligned) {
+        v = *reinterpret_cast<const uint4 *>(col + r0 + 4 * q);
+      } else {
+        if (r0 + 4 * q < n) v.x = col[r0 + 4 * q];
+        if (r0 + 4 * q + 1 < n) v.y = col[r0 + 4 * q + 1];
+        if (r0 + 4 * q + 2 < n) v.z = col[r0 + 4 * q + 2];
+        if (r0 + 4 * q + 3 < n) v.w = col[r0 + 4 * q + 3];
       }
-      const int t = lo - 1;
-      if (valid) keys[r] = starts[t] < x || light[t] ? (uint16_t)t : (uint16_t)bins;
-      bin[c] = valid ? (uint32_t)t : (uint32_t)bins;
+      const CODE xs[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int j = 4 * q + i;
+        const CODE x = xs[i];
+        int lo = 1, hi = bins;  // t = number of k in [1, bins-1] with start[k] <= x
+        while (lo < hi) {
+          const int mid = (lo + hi) >> 1;
+          if (starts[mid] <= x)
+            lo = mid + 1;
+          else
+            hi = mid;
+        }
+        const int t0 = lo - 1;
+        const bool valid = r0 + j < n;
+        const uint32_t t = valid ? (uint32_t)t0 : (uint32_t)bins;
+        const uint32_t key = starts[t0] < x || light[t0] ? (uint32_t)t0 : (uint32_t)bins;
+        if (j & 1)
+          kw[j / 2] |= key << 16;
+        else
+          kw[j / 2] = key;
+#pragma unroll
+        for (int b = 0; b < NB; b++) slice[b] = (slice[b] << 1) | ((t >> b) & 1u);
+      }
+    }
+    if (whole) {
+      uint4 *kd = reinterpret_cast<uint4 *>(keys + r0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) kd[q] = make_uint4(kw[4 * q], kw[4 * q + 1], kw[4 * q + 2], kw[4 * q + 3]);
+    } else {
+#pragma unroll
+      for (int j = 0; j < 32; j++)
+        if (r0 + j < n) keys[r0 + j] = (uint16_t)(kw[j / 2] >> (16 * (j & 1)));
     }
     BITS *dst = fvs + tile * (kFvTileRows / 32) + lane;
+    uint32_t acc = 0;  // fv[k] restricted to this lane's word
     for (int k = 0; k < bins - 1; k++) {
-      uint32_t word = 0;
+      uint32_t eq = ~0u;
 #pragma unroll
-      for (int c = 0; c < kChunks; c++) {
-        const uint64_t b = __ballot(bin[c] <= (uint32_t)k);
-        if (lane == 2 * c) word = __brev((uint32_t)b);
-        if (lane == 2 * c + 1) word = __brev((uint32_t)(b >> 32));
-      }
-      dst[k * fv_stride] = word;
+      for (int b = 0; b < NB; b++) eq &= slice[b] ^ (((uint32_t)k >> b) & 1u ? 0u : ~0u);
+      acc |= eq;
+      dst[k * fv_stride] = acc;
     }
   }
 }
@@ -439,8 +470,21 @@ hipError_t bindex_build(BinDex *bx, const CODE *col, int64_t n, CODE *owned, int
   RTS_TRY(hipMemcpyAsync(light_d, light_h, bins, hipMemcpyHostToDevice, s));
   RTS_TRY(hipMalloc((void **)&d->fvs, (size_t)(bins - 1) * d->fv_stride * sizeof(BITS)));
   RTS_TRY(hipMalloc((void **)&keys, n * sizeof(uint16_t)));
-  hipLaunchKernelGGL(k_build_fvs, dim3(blocks_for((n + kFvTileRows - 1) / kFvTileRows, kBlock / kWave)), dim3(kBlock), 0,
-                     s, col, n, estarts_d, light_d, bins, d->fvs, d->fv_stride, keys);
+  {
+    const dim3 grid(blocks_for((n + kFvTileRows - 1) / kFvTileRows, kBlock / kWave));
+    if (bins <= 128)
+      hipLaunchKernelGGL(k_build_fvs<8>, grid, dim3(kBlock), 0, s, col, n, estarts_d, light_d, bins, d->fvs,
+                         d->fv_stride, keys);
+    else if (bins <= 256)
+      hipLaunchKernelGGL(k_build_fvs<9>, grid, dim3(kBlock), 0, s, col, n, estarts_d, light_d, bins, d->fvs,
+                         d->fv_stride, keys);
+    else if (bins <= 512)
+      hipLaunchKernelGGL(k_build_fvs<10>, grid, dim3(kBlock), 0, s, col, n, estarts_d, light_d, bins, d->fvs,
+                         d->fv_stride, keys);
+    else
+      hipLaunchKernelGGL(k_build_fvs<11>, grid, dim3(kBlock), 0, s, col, n, estarts_d, light_d, bins, d->fvs,
+                         d->fv_stride, keys);
+  }
   RTS_TRY(hipGetLastError());
 
   // 4. B lists: stable sort of (key, row) -> rows grouped by bin in row order
