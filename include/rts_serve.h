@@ -13,9 +13,9 @@
  *
  * The loop's streams share the process's hardware queues (GPU_MAX_HW_QUEUES, 4 by default)
  * with every other stream the process has used, the per-thread stream of a thread that
- * called a reference-named wrapper included; a process that serves should keep the loop's
- * streams and its own few (measured: 27.3k requests/s at saturation alone, 24.9k after one
- * wrapper call on the main thread).
+ * called a reference-named wrapper included, so a serving process should use few streams
+ * besides the loop's (measured: 27.3k requests/s at saturation with the loop alone, 24.9k
+ * after one wrapper call on the main thread).
  */
 #ifndef RTS_SERVE_H_
 #define RTS_SERVE_H_
