@@ -196,6 +196,43 @@ def test_group_size_change_regroups_and_cache_hits_apply_the_cached_shape(eng, t
     check_results(eng, cols, queries, out, counts, stride)
 
 
+class _Offset:
+    """A device pointer `off` bytes into a DeviceBuffer."""
+
+    def __init__(self, buf, off):
+        self.ptr = buf.ptr + off
+
+
+@pytest.mark.parametrize("bins", [128, 1024])
+def test_build_over_a_column_that_is_not_16_byte_aligned(eng, bins):
+    """The filter-vector build loads a lane's 32 rows with 16-byte loads only when the
+    column is 16-byte aligned; a column 4 bytes into its buffer takes the row-by-row loads.
+    Both builds give the same index: filter vectors, refine entries and query results
+    (2048-row tiles with a ragged last one: n is not a multiple of 2048)."""
+    n = 300_017
+    rng = np.random.default_rng(bins + 3)
+    x = rng.integers(0, 2 ** 32, n, dtype=np.uint64).astype(np.uint32)
+    x[rng.choice(n, 40, replace=False)] = 0x1234_5678  # one duplicated value
+    a = eng.DeviceBuffer.from_numpy(x)
+    padded = eng.DeviceBuffer.from_numpy(np.concatenate([np.zeros(1, np.uint32), x]))
+    ia = eng.BinDexIndex(a, n, bins=bins)
+    iu = eng.BinDexIndex(_Offset(padded, 4), n, bins=bins)
+    assert ia.refine_entries() == iu.refine_entries()
+    for k in (0, 1, 63, 126):
+        assert np.array_equal(ia.filter_vector(k), iu.filter_vector(k)), k
+    W = (n + 31) // 32
+    out = eng.DeviceBuffer(4 * W)
+    cnt = eng.DeviceBuffer(8)
+    xs = np.sort(x)
+    for lo, hi in [(0, MAXU), (int(xs[100]), int(xs[200_000])), (0x1234_5678, 0x1234_5678),
+                   (int(xs[n // 2]), int(xs[n // 2 + 1500]))]:
+        m = interval_mask(x, lo, hi)
+        for ix in (ia, iu):
+            eng.bindex_query([ix], [lo], [hi], out, cnt)
+            assert np.array_equal(out.to_numpy(np.uint32, W), pack_rows(m)), (lo, hi)
+            assert int(cnt.to_numpy(np.uint64, 1)[0]) == int(m.sum()), (lo, hi)
+
+
 @pytest.mark.parametrize("threads,order", [(256, 1), (256, 0), (512, 1)])
 def test_groups_are_packed_to_the_fewest_work_items(eng, threads, order):
     """Queries of 1..4 columns, shuffled, none empty: first fit by decreasing column count
