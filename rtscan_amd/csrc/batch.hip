@@ -375,13 +375,35 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   const uint32_t last_mask = tail_mask((uint64_t)n);
   const int lw = tid * kPer;
   unsigned long long *const next_item = sched + xcd * kSchedStride;
+  // item -> (tile, group) without a 64-bit division per item: while the XCD's items fit 32
+  // bits, q = hi32(it * floor(2^32 / ngroups)) is floor(it / ngroups) or one less, fixed
+  // by one compare (2^32 mod ngroups < ngroups keeps the error under one)
+  const bool items32 = items_x <= (int64_t)0xFFFFFFFFu;
+  const uint32_t ng = (uint32_t)ngroups;
+  const uint32_t ng_recip = ng > 1 ? (uint32_t)((1ull << 32) / ng) : 0u;
   __syncthreads();
   // the first item of each workgroup is fixed (j); later ones come from the XCD's counter,
   // so the workgroups of one XCD stay within a few items of each other and share the
   // current tile's slices in L2 instead of drifting over several tiles
   for (int64_t it = j; it < items_x;) {
-    const int64_t t = xcd + kXcds * (it / ngroups);
-    const BatchGroup *g = groups + it % ngroups;
+    int64_t tq;
+    uint32_t gi;
+    if (items32) {  // the item index is the same in every lane: scalar arithmetic
+      const uint32_t u = uniform_u32((uint32_t)it);
+      uint32_t q = ng > 1 ? __umulhi(u, ng_recip) : u;
+      uint32_t r = u - q * ng;
+      if (r >= ng) {
+        q++;
+        r -= ng;
+      }
+      tq = q;
+      gi = r;
+    } else {
+      tq = it / ngroups;
+      gi = (uint32_t)(it % ngroups);
+    }
+    const int64_t t = xcd + kXcds * tq;
+    const BatchGroup *g = groups + gi;
     const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
 
     // 1. Descriptors.  Each wave reads the records of its own refine lists and the slots
