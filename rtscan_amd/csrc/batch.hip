@@ -277,6 +277,9 @@ __device__ __forceinline__ Words<KP> base_words(const BITS *fv_any, uint32_t bas
 __device__ __forceinline__ uint32_t uniform_u32(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  return ((uint64_t)uniform_u32((uint32_t)(v >> 32)) << 32) | uniform_u32((uint32_t)v);
+}
 template <class T>
 __device__ __forceinline__ T *uniform_ptr(T *p) {
   const uint64_t a = (uint64_t)p;
@@ -380,7 +383,7 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
   // by one compare (2^32 mod ngroups < ngroups keeps the error under one)
   const bool items32 = items_x <= (int64_t)0xFFFFFFFFu;
   const uint32_t ng = (uint32_t)ngroups;
-  const uint32_t ng_recip = ng > 1 ? (uint32_t)((1ull << 32) / ng) : 0u;
+  const uint32_t ng_recip = uniform_u32(ng > 1 ? (uint32_t)((1ull << 32) / ng) : 0u);
   __syncthreads();
   // the first item of each workgroup is fixed (j); later ones come from the XCD's counter,
   // so the workgroups of one XCD stay within a few items of each other and share the
@@ -402,8 +405,9 @@ __global__ __launch_bounds__(NT) __attribute__((amdgpu_waves_per_eu(4))) void k_
       tq = it / ngroups;
       gi = (uint32_t)(it % ngroups);
     }
-    const int64_t t = xcd + kXcds * tq;
-    const BatchGroup *g = groups + gi;
+    // both paths leave scalar values, so the group record and directory loads are scalar
+    const int64_t t = xcd + kXcds * (int64_t)uniform_u64((uint64_t)tq);
+    const BatchGroup *g = groups + uniform_u32(gi);
     const uint32_t row0 = (uint32_t)(t * (int64_t)TW * 32);
 
     // 1. Descriptors.  Each wave reads the records of its own refine lists and the slots
