@@ -79,6 +79,25 @@ def test_batch_matches_closed_form(eng, tile_words, qgroup, direct_share, order,
     plan.free()
 
 
+@pytest.mark.parametrize("nq", [1, 2, 3, 5])
+def test_few_groups_split_items_exactly(eng, nq):
+    """One query per group, so the launch has 1, 2, 3 or 5 groups: a work item's index
+    splits into (tile, group) through the group count's reciprocal, or none for one group;
+    3-tile and 49-tile columns put the items of an XCD on either side of a group boundary."""
+    for n in (3 * 65536 - 7, 49 * 65536 + 100):
+        cols, dcols, idx, sc = make_table(eng, n, seed=nq)
+        queries = random_queries(np.random.default_rng(nq), sc, nq, with_edges=False)
+        plan = eng.BatchPlan(dcols, idx, n, nq, eng.batch_config(2048, 1, 4, 1, 256))
+        plan.set_queries(queries)
+        assert plan.stats()["groups"] == nq
+        stride = ((n + 31) // 32 + 3) // 4 * 4
+        out = eng.DeviceBuffer(4 * stride * nq).fill(0xEE)
+        counts = eng.DeviceBuffer(8 * nq)
+        plan.launch(out, stride, counts)
+        check_results(eng, cols, queries, out, counts, stride)
+        plan.free()
+
+
 def test_batch_without_indexes_is_direct(eng):
     n = 300001
     cols, dcols, idx, sc = make_table(eng, n, seed=5)
